@@ -1,0 +1,105 @@
+/*
+ * koral_hip.h -- C-ABI of the MI355X (gfx950) implementation of KORAL's
+ * per-timestep hot path.  Link libkoral_hip.so; no HIP or torch types cross
+ * this boundary (streams are passed as void*).
+ *
+ * Two families of entry points:
+ *
+ *  1. Batched per-point operators on device-resident SoA arrays.  Point i of
+ *     an n-point batch keeps variable iv at x[iv*n + i]; metric component c at
+ *     g20[c*n + i].  These replace the per-cell functions the reference calls
+ *     in loops (p2u, u2p/calc_primitives, u2p_rad, solve_implicit_lab).
+ *
+ *  2. Grid operators bound to a koral_hip_grid (device-resident state with
+ *     ghosts), replacing op_explicit / op_implicit / calc_u2p and the RK2IMEX
+ *     step body of ko.c.  See the grid section below.
+ *
+ * All calls return 0 on success and a negative KORAL_E_* code on error; the
+ * text of the last error is available from koral_hip_last_error().  Physics
+ * failures are never errors: they surface as cell flags and counters, as in
+ * the reference (mnemonics.h:105-119).
+ */
+#ifndef KORAL_HIP_H
+#define KORAL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KORAL_E_ARG -1
+#define KORAL_E_HIP -2
+#define KORAL_E_STATE -3
+
+/* Resolved compile-time configuration of a KORAL build (choices.h defaults and
+   the problem's define.h).  Mirrors koral_amd/params.py. */
+typedef struct koral_params {
+  int nv, mag, rad, pad0;           /* NV, MAGNFIELD, RADIATION */
+  double gamma;                     /* GAMMA */
+  double k_boltz, m_proton, sigma_rad, mu_gas; /* ko.h:40-46 in code units */
+  double ggg, ccc, masscm;          /* ko.h:9-15 */
+  int kappa_model, pad1;            /* 0: kappa=a*rho; 1: Kramers (cgs coefficient a) */
+  double kappa_a_coef, kappa_es_coef;
+  double rhofloor, uurhoratiomin, uurhoratiomax, b2rhoratiomax;
+  double gammamaxhd, gammamaxrad, eradfloor;
+  double eerhoratiomin, eerhoratiomax, eeuuratiomin, eeuuratiomax;
+  double u2pconv;                   /* U2PCONV */
+  double radimpconv, radimpentrconv, radimpeps;
+  int radimpmaxiter, convrel_enabled; /* RADIMPMAXITER, BHDISK_PROBLEMTYPE */
+  double dampingfactor, maxenchangedown, maxenchangeup, maxradimpdamping;
+  double radimplicitthreshold;
+  double radimpconvrel, radimpconvrelerr, radimpconvrelentr, radimpconvrelentrerr;
+  int int_order, fluxmethod;        /* INT_ORDER (1 minmod, 2 PPM, 4 MP5), FLUXMETHOD */
+  double tsteplim, minmod_theta;
+  int dofixups, dou2pmhdfixups, dou2pradfixups, doradimpfixups;
+} koral_params;
+
+/* Library and device information. */
+const char *koral_hip_last_error(void);
+int koral_hip_version(void);
+int koral_hip_params_size(void);
+
+/* ---- batched per-point operators (device pointers, SoA, asynchronous on
+   `stream`; synchronise before reading results) ---- */
+
+/* p2u (p2u.c:40-49): uu = U(pp) */
+int koral_hip_p2u_batch(const koral_params *P, long n, const double *pp, double *uu,
+                        const double *g20, const double *G20, void *stream);
+
+/* calc_primitives (u2p.c:13-133) on independent points: u2p (u2p.c:176-485)
+   with hot/entropy fallback, M1 inversion, check_floors_mhd/rad, then p2u.
+   uu is updated to follow floored primitives; pp holds the guess on entry.
+   flags[f*n+i] for f in 0..6 (mnemonics.h cell flags); ret[i] = u2p return. */
+int koral_hip_u2p_batch(const koral_params *P, long n, double *uu, double *pp,
+                        const double *g20, const double *G20, int *ret, int *flags,
+                        void *stream);
+
+/* u2p_solver (u2p.c:1219-1235 -> u2p_solver_W u2p.c:1965-2352), etype 0 hot
+   or 1 entropy; pp is the guess on entry; ret[i] is the solver return code. */
+int koral_hip_u2p_solver_batch(const koral_params *P, long n, const double *uu, double *pp,
+                               const double *g20, const double *G20, int etype, int *ret,
+                               void *stream);
+
+/* u2p_rad (u2prad.c:485-511): M1 inversion of R^t_mu; corr[i] = corrected */
+int koral_hip_u2p_rad_batch(const koral_params *P, long n, const double *uu, double *pp,
+                            const double *g20, const double *G20, int *ret, int *corr,
+                            void *stream);
+
+/* calc_Gi (rad.c:1787-1888): contravariant four-force, out[k*n+i] */
+int koral_hip_Gi_batch(const koral_params *P, long n, const double *pp, const double *g20,
+                       const double *G20, int labframe, double *Gi, void *stream);
+
+/* implicit_lab_rad_source_term / solve_implicit_lab (rad.c:2758-2795,
+   rad.c:1225-1570) on independent cells: full six-attempt fallback chain of
+   the damped 4-D Newton solve_implicit_lab_4dprim (rad.c:376-1218).
+   uu/pp updated in place on success; flags as above (RADIMPFIXUPFLAG,
+   RADFIXUPFLAG, RADSOURCETYPEFLAG); info[k*n+i]: k=0 iterations of the
+   successful attempt, 1 attempt index (-1 failed), 2 whichprim, 3 whicheq,
+   4 whichframe, 5 iterations over all attempts. */
+int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *pp,
+                             const double *g20, const double *G20, const double *dt,
+                             double global_time, int *flags, int *info, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""Pins the oracle's implicit gas-radiation solve (solve_implicit_lab,
+rad.c:1225-1570) with properties of the exact solution rather than by
+re-deriving the solver: total conservation, the equations the solution must
+satisfy, and the stiff / dt->0 / zero-opacity limits."""
+import numpy as np
+
+from koral_amd import params as KP, problems as PR
+
+PI = 3.141592654
+
+
+def _batch(n, seed, gamma_max=10.0, **kw):
+    P = KP.params_implicit_batch()
+    for k, v in kw.items():
+        setattr(P, k, v)
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=seed, gamma_max=gamma_max)
+    return P, pp, g20, G20, dt
+
+
+def test_total_energy_momentum_conserved(oracle):
+    P, pp, g20, G20, dt = _batch(600, seed=21)
+    uu = oracle.p2u(P, pp, g20, G20)
+    uo, po, fl, info = oracle.implicit(P, uu, pp, g20, G20, dt)
+    ok = info[:, 1] >= 0
+    assert ok.mean() > 0.9
+    ee = P.ee
+    tot0 = uu[:, 1:5] + uu[:, ee:ee + 4]
+    tot1 = uo[:, 1:5] + uo[:, ee:ee + 4]
+    scale = np.abs(uu[:, 1:5]).max(1) + np.abs(uu[:, ee:ee + 4]).max(1)
+    err = np.abs(tot1 - tot0).max(1) / scale
+    assert np.all(err[ok] < 1e-8), np.sort(err[ok])[-5:]
+    # density and field untouched
+    assert np.allclose(uo[ok, 0], uu[ok, 0], rtol=1e-12)
+    # failures leave the cell unchanged and raise both flags (rad.c:1468-1474)
+    assert np.array_equal(uo[~ok], uu[~ok])
+    assert np.all(fl[~ok, 1] == -1) and np.all(fl[~ok, 3] == -1)
+    assert np.all(fl[ok, 1] == 0) and np.all(fl[ok, 3] == 0)
+
+
+def test_solution_satisfies_source_equation(oracle):
+    """U_rad^new - U_rad^old = -dt sqrt(-g) G_mu(p^new) for cells solved in the
+    lab frame (energy equation), checked with an independent evaluation of
+    the four-force at the returned primitives."""
+    P, pp, g20, G20, dt = _batch(600, seed=22)
+    uu = oracle.p2u(P, pp, g20, G20)
+    uo, po, fl, info = oracle.implicit(P, uu, pp, g20, G20, dt)
+    sel = (info[:, 1] >= 0) & (info[:, 3] == 0) & (info[:, 4] == 0)
+    assert sel.sum() > 300
+    G = oracle.Gi(P, po, g20, G20, 1)
+    g = g20.reshape(-1, 4, 5)[:, :, :4]
+    Gcov = np.einsum("nij,nj->ni", g, G)
+    gdet = g20[:, 19]
+    ee = P.ee
+    lhs = uo[:, ee:ee + 4] - uu[:, ee:ee + 4]
+    rhs = -dt[:, None] * gdet[:, None] * Gcov
+    # the solver's own error norm (rad.c:193-262) is relative to the iterated
+    # half: gas conserved for MHD primitives, radiation for RAD primitives
+    it = np.where((info[:, 2] == 2)[:, None], np.arange(1, 5)[None, :], np.arange(ee, ee + 4)[None, :])
+    iu_new = np.take_along_axis(uo, it, 1)
+    iu_old = np.take_along_axis(uu, it, 1)
+    scale = np.abs(iu_new) + np.abs(iu_old) + np.abs(rhs)
+    rel = (np.abs(lhs - rhs) / scale)[sel]
+    assert rel.max() < 1e-9, np.sort(rel.max(1))[-5:]
+
+
+def test_stiff_limit_reaches_lte(oracle):
+    """dt*kappa -> infinity (1e6..1e9): radiation thermalises with the gas,
+    E_hat = 4 sigma T_gas^4.  Grey absorption (kappa = rho) and a radiation
+    constant scaled so that LTE radiation and gas energies are comparable
+    (as PROBLEMS/RADTUBE/define.h does), so the equilibrium is well posed."""
+    P, pp, g20, G20, _ = _batch(300, seed=23, gamma_max=1.5, kappa_model=0, kappa_a_coef=1.0,
+                                kappa_es_coef=0.0)
+    rng = np.random.default_rng(5)
+    n = len(pp)
+    pp[:, 0] = 10 ** rng.uniform(-1, 1, n)
+    T = 10 ** rng.uniform(6, 7, n)
+    pp[:, 1] = KP.u_from_Trho(P, T, pp[:, 0])
+    P.sigma_rad = np.median(pp[:, 1]) / (4. * np.median(T) ** 4)
+    pp[:, P.ee] = pp[:, 1] * 10 ** rng.uniform(-1, 1, n)
+    pp[:, 5] = pp[:, 0] * np.log(((P.gamma - 1.) * pp[:, 1]) ** 1.5 / pp[:, 0] ** 2.5)
+    kap = PR.kappa_abs_np(P, pp[:, 0], pp[:, 1])
+    dt = np.exp(rng.uniform(np.log(1e6), np.log(1e9), n)) / kap
+    uu = oracle.p2u(P, pp, g20, G20)
+    uo, po, fl, info = oracle.implicit(P, uu, pp, g20, G20, dt)
+    ok = info[:, 1] >= 0
+    assert ok.mean() > 0.75
+    T1 = KP.T_from_urho(P, po[:, 1], po[:, 0])
+    Ehat = -oracle.ff_Rtt(P, po, g20, G20)
+    Elte = 4. * P.sigma_rad * T1 ** 4
+    rel = np.abs(Ehat - Elte) / (Ehat + Elte)
+    assert np.quantile(rel[ok], 0.95) < 1e-5, np.sort(rel[ok])[-5:]
+
+
+def test_small_dt_is_identity(oracle):
+    P, pp, g20, G20, dt = _batch(200, seed=24)
+    kap = PR.kappa_abs_np(P, pp[:, 0], pp[:, 1])
+    dt = 1e-14 / kap
+    uu = oracle.p2u(P, pp, g20, G20)
+    uo, po, fl, info = oracle.implicit(P, uu, pp, g20, G20, dt)
+    ok = info[:, 1] >= 0
+    assert ok.all()
+    assert np.allclose(po[:, :5], pp[:, :5], rtol=1e-8)
+    # gas-primitive cells recompute U_rad = U_rad0 - dU_gas, so rounding of the
+    # (rest-mass dominated) gas energy bounds how well U_rad is reproduced
+    scale = np.abs(uu[:, 1:5]).max(1) + np.abs(uu[:, P.ee:P.ee + 4]).max(1)
+    keep = [i for i in range(P.nv) if i != 5]  # entropy is recomputed from (rho, u)
+    assert (np.abs(uo - uu)[:, keep].max(1) / scale < 1e-13).all()
+    assert np.all(info[:, 0] == 1)  # converged on the initial state
+
+
+def test_zero_opacity_is_identity(oracle):
+    P, pp, g20, G20, dt = _batch(200, seed=25)
+    P.kappa_a_coef = 0.0
+    P.kappa_es_coef = 0.0
+    uu = oracle.p2u(P, pp, g20, G20)
+    uo, po, fl, info = oracle.implicit(P, uu, pp, g20, G20, dt)
+    assert np.all(info[:, 1] == 0) and np.all(info[:, 0] == 1)
+    scale = np.abs(uu[:, 1:5]).max(1) + np.abs(uu[:, P.ee:P.ee + 4]).max(1)
+    keep = [i for i in range(P.nv) if i != 5]
+    assert (np.abs(uo - uu)[:, keep].max(1) / scale < 1e-13).all()
