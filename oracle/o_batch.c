@@ -140,3 +140,37 @@ void oracle_implicit_batch(const oracle_params *P, long n, double *uu, double *p
     o[5] = inf.total_iters;
   }
 }
+
+/* ---- grid-level entry points (o_grid.c) ---- */
+#include "o_grid.h"
+
+int oracle_grid_set_bc(const oracle_params *P, const o_grid *G, double *u, double *p)
+{
+  o_set_bc(P, G, u, p);
+  return 0;
+}
+
+double oracle_grid_tstepden(const oracle_params *P, const o_grid *G, const double *p) { return o_tstepden(P, G, p); }
+
+int oracle_grid_explicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                         double *tstepdenmax)
+{
+  return o_op_explicit(P, G, t, dt, u, p, tstepdenmax);
+}
+
+int oracle_grid_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts)
+{
+  return o_calc_u2p(P, G, u, p, flags, counts);
+}
+
+int oracle_grid_implicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                         int *flags, int *counts)
+{
+  return o_op_implicit(P, G, t, dt, u, p, flags, counts);
+}
+
+int oracle_grid_step(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
+                     double *tstepdenmax, int *counts)
+{
+  return o_step_rk2imex(P, G, t, dt, u, p, flags, tstepdenmax, counts);
+}

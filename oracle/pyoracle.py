@@ -149,3 +149,98 @@ def inverse44(a):
     ia = np.zeros(16)
     r = lib().oracle_inverse44(_d(a), _d(ia))
     return r, ia.reshape(4, 4)
+
+
+# ---- grid-level operators ----
+class OGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
+                ("ngx", ctypes.c_int), ("ngy", ctypes.c_int), ("ngz", ctypes.c_int),
+                ("sx", ctypes.c_int), ("sy", ctypes.c_int), ("sz", ctypes.c_int),
+                ("bc", ctypes.c_int * 6),
+                ("xb", _dp * 3),
+                ("gc", _dp), ("Gc", _dp), ("kr", _dp), ("gfx", _dp), ("Gfx", _dp), ("gfy", _dp), ("Gfy", _dp)]
+
+
+class Grid:
+    """Wraps a koral_amd.grid.GridSpec for the oracle (keeps arrays alive)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.t = spec.aos_tables()
+        self.xb = [np.ascontiguousarray(x, dtype=np.float64) for x in spec.xb]
+        g = OGrid()
+        for k in ("nx", "ny", "nz", "ng", "ngx", "ngy", "ngz", "sx", "sy", "sz"):
+            setattr(g, k, getattr(spec, k))
+        for i in range(6):
+            g.bc[i] = spec.bc[i]
+        for d in range(3):
+            g.xb[d] = _d(self.xb[d])
+        for k in ("gc", "Gc", "kr", "gfx", "Gfx", "gfy", "Gfy"):
+            setattr(g, k, _d(self.t[k]))
+        self.c = g
+
+
+def _grid_lib():
+    L = lib()
+    if not hasattr(L, "_grid_sigs"):
+        vp = ctypes.c_void_p
+        L.oracle_grid_set_bc.argtypes = [vp, vp, _dp, _dp]
+        L.oracle_grid_tstepden.argtypes = [vp, vp, _dp]
+        L.oracle_grid_tstepden.restype = ctypes.c_double
+        L.oracle_grid_explicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_grid_u2p.argtypes = [vp, vp, _dp, _dp, _ip, _ip]
+        L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
+        L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
+        L._grid_sigs = True
+    return L
+
+
+class GridState:
+    """AoS state u, p [z, y, x, nv] and flags [z, y, x, 7] on an oracle grid."""
+
+    def __init__(self, P, grid, p_aos, u_aos=None, flags=None):
+        self.P, self.grid = P, grid
+        self.p = np.ascontiguousarray(p_aos, dtype=np.float64).copy()
+        self.u = np.zeros_like(self.p) if u_aos is None else np.ascontiguousarray(u_aos, dtype=np.float64).copy()
+        self.flags = np.zeros(self.p.shape[:-1] + (7,), np.int32) if flags is None else flags.copy()
+
+    def init_u(self):
+        """u = p2u(p) in every cell (as set_initial_profile does)."""
+        gc, Gc = self.grid.spec.cell_tables()
+        nv = self.P.nv
+        self.u[...] = p2u(self.P, self.p.reshape(-1, nv), np.ascontiguousarray(gc).reshape(-1, 20),
+                          np.ascontiguousarray(Gc).reshape(-1, 20)).reshape(self.p.shape)
+
+    def set_bc(self):
+        _grid_lib().oracle_grid_set_bc(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p))
+
+    def tstepden(self):
+        return _grid_lib().oracle_grid_tstepden(_P(self.P), ctypes.byref(self.grid.c), _d(self.p))
+
+    def explicit(self, t, dt):
+        tm = np.array([-1.0])
+        _grid_lib().oracle_grid_explicit(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
+                                         _d(tm))
+        return tm[0]
+
+    def u2p(self):
+        c = np.zeros(3, np.int32)
+        _grid_lib().oracle_grid_u2p(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p),
+                                    _i(self.flags), _i(c))
+        return c
+
+    def implicit(self, t, dt):
+        c = np.zeros(3, np.int32)
+        _grid_lib().oracle_grid_implicit(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
+                                         _i(self.flags), _i(c))
+        return c
+
+    def step(self, t, dt):
+        """One RK2IMEX step; returns the next tstepdenmax and counters
+        [implicit successes, implicit iterations, implicit failures,
+         hd fixups, rad fixups, entropy inversions]."""
+        tm = np.array([-1.0])
+        c = np.zeros(6, np.int32)
+        _grid_lib().oracle_grid_step(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
+                                     _i(self.flags), _d(tm), _i(c))
+        return tm[0], c
