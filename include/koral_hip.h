@@ -99,6 +99,60 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
                              const double *g20, const double *G20, const double *dt,
                              double global_time, int *flags, int *info, void *stream);
 
+
+/* ---- grid operators (device-resident state, one GPU holding the whole
+   domain that KORAL's MPI tiling would split) ---- */
+
+/* Grid description: interior sizes (1 = collapsed dimension, no ghosts), NG
+   ghosts, boundary kind per side (0 copy, 1 periodic, 2 polar reflection of
+   the normal vector components) ordered xlo,xhi,ylo,yhi,zlo,zhi, face
+   coordinates xb[d][i+ng] for i in [-ng, n+ng], and host SoA metric tables
+   over the ghost-inclusive (x1,x2) plane (KORAL's calc_metric output,
+   metric.c:3866-4080, for an x3-independent metric):
+     gc, Gc [20][SY][SX]   cell centres (struct geometry gg/GG)
+     kr     [64][SY][SX]   Christoffel symbols Gamma^i_jk at centres
+     gfx,Gfx [20][SY][SX+1], gfy,Gfy [20][SY+1][SX]   face metrics */
+typedef struct koral_grid_desc {
+  int nx, ny, nz, ng;
+  int bc[6];
+  const double *xb[3];
+  const double *gc, *Gc, *kr, *gfx, *Gfx, *gfy, *Gfy;
+} koral_grid_desc;
+
+typedef struct koral_hip_grid koral_hip_grid;
+
+int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral_hip_grid **out);
+int koral_hip_grid_destroy(koral_hip_grid *g);
+int koral_hip_grid_set_stream(koral_hip_grid *g, void *stream);
+
+/* Parity-mode boundary: KORAL's host arrays (ko.h:505-520 layout,
+   [iz][iy][ix][iv] over the ghost-inclusive box; cellflag [..][7] ints).
+   u and flags may be NULL. */
+int koral_hip_grid_upload(koral_hip_grid *g, const double *p_aos, const double *u_aos, const int *flags_aos);
+int koral_hip_grid_download(koral_hip_grid *g, double *p_aos, double *u_aos, int *flags_aos);
+/* Resident mode: device SoA state x[iv][cell], flags int8 [7][cell]. */
+int koral_hip_grid_device_state(koral_hip_grid *g, double **p, double **u, signed char **flags);
+
+/* u = p2u(p) in every cell (as set_initial_profile leaves the state) */
+int koral_hip_grid_p2u(koral_hip_grid *g);
+/* set_bc (finite.c:2752-3700): ghost cells from the boundary kinds, then the
+   domain corners */
+int koral_hip_set_bc(koral_hip_grid *g);
+/* calc_wavespeeds + save_wavespeeds reduction (finite.c:296-398, 568-599):
+   returns max over the domain of the time-step density */
+int koral_hip_tstepden(koral_hip_grid *g, double *tstepdenmax);
+/* op_explicit (finite.c:647-1234); tstepdenmax may be NULL */
+int koral_hip_op_explicit(koral_hip_grid *g, double t, double dt, double *tstepdenmax);
+/* calc_u2p (finite.c:502-561) including cell_fixup and set_bc */
+int koral_hip_calc_u2p(koral_hip_grid *g);
+/* op_implicit (finite.c:1242-1292) including cell_fixup(FIXUP_RADIMP) */
+int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt);
+/* Body of one RK2IMEX step of ko.c:444-570.  tstepdenmax (next dt = 1/it)
+   and counters[6] = {implicit successes, implicit Newton iterations,
+   implicit failures, hd fixup requests, rad fixup requests, entropy
+   inversions} may be NULL. */
+int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstepdenmax, int *counters);
+
 #ifdef __cplusplus
 }
 #endif

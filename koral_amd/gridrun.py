@@ -1,0 +1,148 @@
+"""Host-side driver of the device grid operators (the Python mirror of ko.c's
+outer loop around the C-ABI in include/koral_hip.h).
+
+    dg = DeviceGrid(P, spec)          # uploads metric tables, allocates state
+    dg.upload(p_aos)                  # KORAL host layout [z, y, x, nv]
+    dg.init()                         # u = p2u(p), set_bc, first dt
+    dg.step()                         # one RK2IMEX step, dt from the last one
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
+                ("bc", ctypes.c_int * 6), ("xb", _dp * 3),
+                ("gc", _dp), ("Gc", _dp), ("kr", _dp), ("gfx", _dp), ("Gfx", _dp), ("gfy", _dp), ("Gfy", _dp)]
+
+
+def _sig(L):
+    if getattr(L, "_grid_sigs", False):
+        return L
+    i = ctypes.c_int
+    d = ctypes.c_double
+    for name, args in {
+        "koral_hip_grid_create": [_vp, _vp, ctypes.POINTER(_vp)],
+        "koral_hip_grid_destroy": [_vp],
+        "koral_hip_grid_set_stream": [_vp, _vp],
+        "koral_hip_grid_upload": [_vp, _vp, _vp, _vp],
+        "koral_hip_grid_download": [_vp, _vp, _vp, _vp],
+        "koral_hip_grid_device_state": [_vp, _vp, _vp, _vp],
+        "koral_hip_grid_p2u": [_vp],
+        "koral_hip_set_bc": [_vp],
+        "koral_hip_tstepden": [_vp, _vp],
+        "koral_hip_op_explicit": [_vp, d, d, _vp],
+        "koral_hip_calc_u2p": [_vp],
+        "koral_hip_op_implicit": [_vp, d, d],
+        "koral_hip_step_rk2imex": [_vp, d, d, _vp, _vp],
+    }.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i
+    L._grid_sigs = True
+    return L
+
+
+class DeviceGrid:
+    def __init__(self, P, spec, stream=None):
+        self.P, self.spec = P, spec
+        self.L = _sig(lib())
+        t = spec.tables
+        self._keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in t.items()}
+        self._xb = [np.ascontiguousarray(x, dtype=np.float64) for x in spec.xb]
+        D = GridDesc()
+        D.nx, D.ny, D.nz, D.ng = spec.nx, spec.ny, spec.nz, spec.ng
+        for k in range(6):
+            D.bc[k] = spec.bc[k]
+        for k in range(3):
+            D.xb[k] = self._xb[k].ctypes.data_as(_dp)
+        for k in ("gc", "Gc", "kr", "gfx", "Gfx", "gfy", "Gfy"):
+            setattr(D, k, self._keep[k].ctypes.data_as(_dp))
+        self.desc = D
+        h = _vp()
+        check(self.L.koral_hip_grid_create(ctypes.byref(P), ctypes.byref(D), ctypes.byref(h)), "grid_create")
+        self.h = h
+        if stream is not None:
+            check(self.L.koral_hip_grid_set_stream(self.h, _vp(stream)), "set_stream")
+        self.t = 0.0
+        self.dt = None
+        self.counters = np.zeros(6, np.int32)
+
+    def close(self):
+        if self.h:
+            self.L.koral_hip_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape_aos(self):
+        s = self.spec
+        return (s.sz, s.sy, s.sx, self.P.nv)
+
+    def upload(self, p_aos, u_aos=None, flags=None):
+        p = np.ascontiguousarray(p_aos, dtype=np.float64)
+        u = None if u_aos is None else np.ascontiguousarray(u_aos, dtype=np.float64)
+        f = None if flags is None else np.ascontiguousarray(flags, dtype=np.int32)
+        check(self.L.koral_hip_grid_upload(self.h, p.ctypes.data_as(_vp),
+                                           None if u is None else u.ctypes.data_as(_vp),
+                                           None if f is None else f.ctypes.data_as(_vp)), "grid_upload")
+
+    def download(self):
+        p = np.zeros(self.shape_aos)
+        u = np.zeros(self.shape_aos)
+        f = np.zeros(self.shape_aos[:-1] + (7,), np.int32)
+        check(self.L.koral_hip_grid_download(self.h, p.ctypes.data_as(_vp), u.ctypes.data_as(_vp),
+                                             f.ctypes.data_as(_vp)), "grid_download")
+        return p, u, f
+
+    def p2u(self):
+        check(self.L.koral_hip_grid_p2u(self.h), "grid_p2u")
+
+    def set_bc(self):
+        check(self.L.koral_hip_set_bc(self.h), "set_bc")
+
+    def tstepden(self):
+        v = ctypes.c_double(0.)
+        check(self.L.koral_hip_tstepden(self.h, ctypes.byref(v)), "tstepden")
+        return v.value
+
+    def explicit(self, t, dt):
+        v = ctypes.c_double(0.)
+        check(self.L.koral_hip_op_explicit(self.h, t, dt, ctypes.byref(v)), "op_explicit")
+        return v.value
+
+    def u2p(self):
+        check(self.L.koral_hip_calc_u2p(self.h), "calc_u2p")
+
+    def implicit(self, t, dt):
+        check(self.L.koral_hip_op_implicit(self.h, t, dt), "op_implicit")
+
+    def init(self):
+        """State as left by KORAL's initialisation: u = p2u(p), ghosts filled,
+        first dt from the wave speeds."""
+        self.p2u()
+        self.set_bc()
+        self.dt = 1. / self.tstepden()
+
+    def step(self, sync=True):
+        """One RK2IMEX step with dt from the previous step (ko.c:399)."""
+        tsd = ctypes.c_double(0.)
+        c = self.counters
+        check(self.L.koral_hip_step_rk2imex(self.h, self.t, self.dt,
+                                            ctypes.byref(tsd) if sync else None,
+                                            c.ctypes.data_as(_vp) if sync else None), "step_rk2imex")
+        self.t += self.dt
+        if sync:
+            self.dt = 1. / tsd.value
+        return self.dt

@@ -69,6 +69,11 @@ def kappa_cgs2gu(x, masscm):
     return x / GGG * CCC * CCC / masscm
 
 
+def rho_cgs2gu(x, masscm):
+    """rhoCGS2GU, ko.h:25"""
+    return x * GGG / CCC / CCC * masscm * masscm
+
+
 def base_params(mag, rad, gamma, mass_msun=10.0, masscm=None):
     """KORAL defaults (choices.h) for the given layout."""
     P = KoralParams()

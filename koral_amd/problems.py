@@ -6,6 +6,7 @@ and time steps that the device kernels then consume.
 import numpy as np
 
 from . import metric as M
+from . import params as KP
 from .params import u_from_Trho, T_from_urho
 
 
@@ -172,20 +173,27 @@ def tube_setup(P, nx=512, seed=0, xlim=(-15., 15.)):
 
 
 def torus_setup(P, n=(128, 128, 64), spin=0.9, rin=6.0, rmax=12.0, r_grid=(1.2, 60.), th0=0.02 * np.pi,
-                beta=100., seed=0, pert=0.02, rad_frac=0.1):
+                beta=100., seed=0, pert=0.02, rad_frac=0.1, rho_max=None):
     """Magnetised constant-angular-momentum torus around a Kerr black hole in
     MKS1 coordinates (r = exp x1, theta = x2, phi = x3).
 
     l = l_K(rmax) (Keplerian at the pressure maximum); enthalpy from
     W = ln(-u_t) with the surface through (rin, pi/2); polytrope p = K rho^Gamma
-    normalised to rho_max = 1.  Atmosphere rho = 1e-4 r^-1.5, u = 1e-6 r^-2.5,
-    normal-observer velocity.  Seeded uniform +-pert relative noise on u.
+    normalised to rho = 1 at the density maximum.  Atmosphere rho = 1e-4 r^-1.5,
+    u = 1e-6 r^-2.5, normal-observer velocity (relele.c:598-625, atmtype 0).
+    Density and energy of torus and atmosphere are then multiplied by rho_max
+    (code units); ideal MHD is invariant under that scaling (B follows from
+    beta), radiation coupling is not.  Default: 1 without radiation; with
+    radiation the density maximum is 3e-2 g/cm^3, the scale of KORAL's
+    radiative torus set-up (PROBLEMS/KATOTORUS/define.h:219, KT_RHO0).  Seeded uniform +-pert relative noise on u.
     A_phi = max(rho/rho_max - 0.2, 0) at cell corners, B^i = curl A / sqrt(-g),
     scaled so that max(p_gas)/max(p_mag) = beta.  With radiation: E_rf =
     rad_frac * u, comoving with the gas.  Boundaries: copy in r, reflection at
     the polar axis, periodic in phi.
     """
     from .grid import GridSpec, ng_for, BC_COPY, BC_PERIODIC, BC_REFLECT
+    if rho_max is None:
+        rho_max = KP.rho_cgs2gu(3e-2, P.masscm) if P.rad else 1.0
     a = spin
     nx, ny, nz = n
     ng = ng_for(P.int_order)
@@ -238,8 +246,8 @@ def torus_setup(P, n=(128, 128, 64), spin=0.9, rin=6.0, rmax=12.0, r_grid=(1.2, 
     u_a = 1e-6 * r ** -2.5
     use_t = inside & (rho_t > rho_a)
     rng = np.random.default_rng(seed)
-    rho = np.where(use_t, rho_t, rho_a)
-    u = np.where(use_t, u_t, u_a) * (1. + pert * rng.uniform(-1., 1., r.shape))
+    rho = np.where(use_t, rho_t, rho_a) * rho_max
+    u = np.where(use_t, u_t, u_a) * (1. + pert * rng.uniform(-1., 1., r.shape)) * rho_max
 
     gc, Gc = spec.cell_tables()
     g = gc.reshape(gc.shape[:-1] + (4, 5))[..., :4]
