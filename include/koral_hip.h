@@ -133,6 +133,14 @@ int koral_hip_grid_download(koral_hip_grid *g, double *p_aos, double *u_aos, int
 /* Resident mode: device SoA state x[iv][cell], flags int8 [7][cell]. */
 int koral_hip_grid_device_state(koral_hip_grid *g, double **p, double **u, signed char **flags);
 
+/* Diagnostics: the last op_explicit's work arrays, device SoA copied to the
+   host.  which = 0: wave speeds [18][cell] (per dimension d: gas min/max at
+   2d, 2d+1, radiation min/max at 6+2d, 7+2d, gas and radiation max |a| at
+   12+d, 15+d); which = 1, 2, 3: face fluxes of direction x, y, z after
+   flux-CT, [nv][face] with faces (SX+1)*SY*SZ, SX*(SY+1)*SZ, SX*SY*(SZ+1).
+   No reference counterpart; used to localise parity differences. */
+int koral_hip_grid_download_work(koral_hip_grid *g, int which, double *out);
+
 /* u = p2u(p) in every cell (as set_initial_profile leaves the state) */
 int koral_hip_grid_p2u(koral_hip_grid *g);
 /* set_bc (finite.c:2752-3700): ghost cells from the boundary kinds, then the

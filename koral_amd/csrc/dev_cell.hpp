@@ -9,7 +9,7 @@
 // built with -ffp-contract=off, so results agree with a CPU FP64 build to
 // rounding of the transcendental functions.
 #pragma once
-#include <hip/hip_runtime.h>
+#include "dev_qual.hpp"
 #include <math.h>
 #include <float.h>
 #include "koral_hip.h"
@@ -40,13 +40,13 @@ struct Geom {
   double gdet, alpha, gttpert;
 };
 
-__device__ __forceinline__ double mx(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double sgn(double x) { return x > 0. ? 1. : (x < 0. ? -1. : 1.); }
+KD_INL double mx(double a, double b) { return a > b ? a : b; }
+KD_INL double sgn(double x) { return x > 0. ? 1. : (x < 0. ? -1. : 1.); }
 
 // fill_geometry, metric.c:1643-1676.  src(c) returns component c of the 20
 // covariant numbers, srcG(c) of the 20 contravariant ones.
 template <class F1, class F2>
-__device__ __forceinline__ void load_geom(Geom &g, F1 src, F2 srcG)
+KD_INL void load_geom(Geom &g, F1 src, F2 srcG)
 {
 #pragma unroll
   for (int c = 0; c < 20; c++) {
@@ -59,7 +59,7 @@ __device__ __forceinline__ void load_geom(Geom &g, F1 src, F2 srcG)
 }
 
 // indices_21 (frames.c:2086-2106) and indices_12 (frames.c:2060-2080)
-__device__ __forceinline__ void lower(const double *a, double *b, const Geom &g)
+KD_INL void lower(const double *a, double *b, const Geom &g)
 {
   double t[4];
 #pragma unroll
@@ -72,7 +72,7 @@ __device__ __forceinline__ void lower(const double *a, double *b, const Geom &g)
   for (int i = 0; i < 4; i++) b[i] = t[i];
 }
 
-__device__ __forceinline__ void raise_(const double *a, double *b, const Geom &g)
+KD_INL void raise_(const double *a, double *b, const Geom &g)
 {
   double t[4];
 #pragma unroll
@@ -85,12 +85,12 @@ __device__ __forceinline__ void raise_(const double *a, double *b, const Geom &g
   for (int i = 0; i < 4; i++) b[i] = t[i];
 }
 
-__device__ __forceinline__ double dot4(const double *a, const double *b)
+KD_INL double dot4(const double *a, const double *b)
 {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
-__device__ __forceinline__ double qsq_of(double v1, double v2, double v3, const Geom &g)
+KD_INL double qsq_of(double v1, double v2, double v3, const Geom &g)
 {
   const double v[4] = {0., v1, v2, v3};
   double q = 0.;
@@ -102,7 +102,7 @@ __device__ __forceinline__ double qsq_of(double v1, double v2, double v3, const 
 }
 
 // VELR -> VEL4 (conv_vels_both, relele.c:105-130; core relele.c:322-338)
-__device__ __forceinline__ void relr_to_4vel(double v1, double v2, double v3, double *ucon, double *ucov,
+KD_INL void relr_to_4vel(double v1, double v2, double v3, double *ucon, double *ucov,
                                              const Geom &g)
 {
   double qsq = qsq_of(v1, v2, v3, g);
@@ -117,7 +117,7 @@ __device__ __forceinline__ void relr_to_4vel(double v1, double v2, double v3, do
 }
 
 // calc_utp1 (VELR), p2u.c:548-575
-__device__ __forceinline__ double utp1(double v1, double v2, double v3, const Geom &g)
+KD_INL double utp1(double v1, double v2, double v3, const Geom &g)
 {
   double qsq = qsq_of(v1, v2, v3, g);
   double gamma2 = 1. + qsq;
@@ -132,7 +132,7 @@ __device__ __forceinline__ double utp1(double v1, double v2, double v3, const Ge
 }
 
 // calc_bcon_4vel, magn.c:7-23
-__device__ __forceinline__ void bcon4(double B1, double B2, double B3, const double *ucon, const double *ucov,
+KD_INL void bcon4(double B1, double B2, double B3, const double *ucon, const double *ucov,
                                       double *bcon)
 {
   bcon[0] = B1 * ucov[1] + B2 * ucov[2] + B3 * ucov[3];
@@ -142,7 +142,7 @@ __device__ __forceinline__ void bcon4(double B1, double B2, double B3, const dou
 }
 
 // calc_Sfromu, physics.c:907-917
-__device__ __forceinline__ double Sfromu(const koral_params &P, double rho, double u)
+KD_INL double Sfromu(const koral_params &P, double rho, double u)
 {
   double gm1 = P.gamma - 1.;
   double n = 1.0 / gm1;
@@ -151,7 +151,7 @@ __device__ __forceinline__ double Sfromu(const koral_params &P, double rho, doub
 
 // p2u_mhd, p2u.c:56-159
 template <int MAG, int RAD>
-__device__ __forceinline__ void p2u_mhd(const koral_params &P, const double *pp, double *uu, const Geom &g)
+KD_INL void p2u_mhd(const koral_params &P, const double *pp, double *uu, const Geom &g)
 {
   double gdetu = g.gdet;
   double rho = pp[0], u = pp[1];
@@ -191,7 +191,7 @@ __device__ __forceinline__ void p2u_mhd(const koral_params &P, const double *pp,
 
 // p2u_rad (M1), p2u.c:253-307
 template <int MAG, int RAD>
-__device__ __forceinline__ void p2u_rad(const double *pp, double *uu, const Geom &g)
+KD_INL void p2u_rad(const double *pp, double *uu, const Geom &g)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
   double gdetu = g.gdet;
@@ -207,7 +207,7 @@ __device__ __forceinline__ void p2u_rad(const double *pp, double *uu, const Geom
 }
 
 template <int MAG, int RAD>
-__device__ __forceinline__ void p2u(const koral_params &P, const double *pp, double *uu, const Geom &g)
+KD_INL void p2u(const koral_params &P, const double *pp, double *uu, const Geom &g)
 {
   p2u_mhd<MAG, RAD>(P, pp, uu, g);
   if (RAD) p2u_rad<MAG, RAD>(pp, uu, g);
@@ -221,7 +221,7 @@ struct U2PCons {
   double Qn, Qtsq, D, QdotBsq, Bsq, Sc;
 };
 
-__device__ __forceinline__ void f_u2p(const koral_params &P, int etype, double Wp, const U2PCons &c, double &f,
+KD_INL void f_u2p(const koral_params &P, int etype, double Wp, const U2PCons &c, double &f,
                                       double &df, double &err)
 {
   const double G = P.gamma, GM1 = P.gamma - 1.;
@@ -269,13 +269,13 @@ __device__ __forceinline__ void f_u2p(const koral_params &P, int etype, double W
   }
 }
 
-__device__ __forceinline__ bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
+KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 {
   return (W * W * W * (W + 2. * Bsq) - QdotBsq * (2. * W + Bsq)) <= W * W * (Qtsq - Bsq * Bsq);
 }
 
 template <int MAG, int RAD>
-__device__ int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype)
+KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype)
 {
   double gdetu = g.gdet;
   double alpha = sqrt(-1. / g.G[0][0]);
@@ -397,7 +397,7 @@ __device__ int u2p_solver_W(const koral_params &P, const double *uu, double *pp,
 // ---------------------------------------------------------------------------
 // M1 closure inversion, u2prad.c:9-481
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void m1_inv(const double *Av, const Geom &g, double &S, double &Sig)
+KD_INL void m1_inv(const double *Av, const Geom &g, double &S, double &Sig)
 {
   S = g.G[0][1] * Av[1] + g.G[0][2] * Av[2] + g.G[0][3] * Av[3];
   double q = 0.;
@@ -409,7 +409,7 @@ __device__ __forceinline__ void m1_inv(const double *Av, const Geom &g, double &
 }
 
 // get_m1closure_gammarel2_cold, u2prad.c:9-102
-__device__ __forceinline__ void m1_cold(const Geom &g, const double *Avc_in, double gammarel2, double &Av0,
+KD_INL void m1_cold(const Geom &g, const double *Avc_in, double gammarel2, double &Av0,
                                         double &Erf, double *urf)
 {
   double alpha = g.alpha;
@@ -434,7 +434,7 @@ __device__ __forceinline__ void m1_cold(const Geom &g, const double *Avc_in, dou
 }
 
 template <int MAG, int RAD>
-__device__ int u2p_rad(const koral_params &P, const double *uu, double *pp, const Geom &g, int &corrected)
+KD_FUN int u2p_rad(const koral_params &P, const double *uu, double *pp, const Geom &g, int &corrected)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
   double gdetu = g.gdet;
@@ -495,7 +495,7 @@ __device__ int u2p_rad(const koral_params &P, const double *uu, double *pp, cons
 // ---------------------------------------------------------------------------
 // calc_Rij_M1, rad.c:1944-1975
 template <int MAG, int RAD>
-__device__ __forceinline__ void Rij_M1(const double *pp, const Geom &g, double R[4][4])
+KD_INL void Rij_M1(const double *pp, const Geom &g, double R[4][4])
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
   double E = pp[EE];
@@ -508,14 +508,14 @@ __device__ __forceinline__ void Rij_M1(const double *pp, const Geom &g, double R
 }
 
 // calc_PEQ_Tfromurho, physics.c:984-989
-__device__ __forceinline__ double T_from_urho(const koral_params &P, double u, double rho)
+KD_INL double T_from_urho(const koral_params &P, double u, double rho)
 {
   double p = u * (P.gamma - 1.);
   return p / (P.k_boltz * rho / P.mu_gas / P.m_proton);
 }
 
 // calc_kappa with PROBLEMS/RADTUBE/kappa.c or PROBLEMS/KATOTORUS/kappa.c
-__device__ __forceinline__ double kappa_abs(const koral_params &P, double rho, double u)
+KD_INL double kappa_abs(const koral_params &P, double rho, double u)
 {
   if (P.kappa_model == 0) return P.kappa_a_coef * rho;
   double T = T_from_urho(P, u, rho);
@@ -525,11 +525,11 @@ __device__ __forceinline__ double kappa_abs(const koral_params &P, double rho, d
   return kcgs / P.ggg * P.ccc * P.ccc / P.masscm * rho;
 }
 
-__device__ __forceinline__ double kappa_es(const koral_params &P, double rho) { return P.kappa_es_coef * rho; }
+KD_INL double kappa_es(const koral_params &P, double rho) { return P.kappa_es_coef * rho; }
 
 // calc_Gi, rad.c:1787-1888
 template <int MAG, int RAD>
-__device__ __forceinline__ void calc_Gi(const koral_params &P, const double *pp, const Geom &g, double *Gi,
+KD_INL void calc_Gi(const koral_params &P, const double *pp, const Geom &g, double *Gi,
                                         int labframe)
 {
   double R[4][4];
@@ -559,7 +559,7 @@ __device__ __forceinline__ void calc_Gi(const koral_params &P, const double *pp,
 
 // calc_ff_Rtt, rad.c:2833-2862
 template <int MAG, int RAD>
-__device__ __forceinline__ double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
+KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 {
   double uc[4], uv[4];
   relr_to_4vel(pp[2], pp[3], pp[4], uc, uv, g);
@@ -587,7 +587,7 @@ __device__ __forceinline__ double ff_Rtt(const double *pp, const Geom &g, double
 // ---------------------------------------------------------------------------
 // u2p, u2p.c:176-485 (type 0)
 template <int MAG, int RAD>
-__device__ int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
+KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   double uu[NV], ppbak[NV];
@@ -629,7 +629,7 @@ __device__ int u2p(const koral_params &P, const double *uu0, double *pp, const G
 
 // check_floors_mhd, u2p.c:492-695
 template <int MAG, int RAD>
-__device__ int floors_mhd(const koral_params &P, double *pp, const Geom &g)
+KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   int ret = 0;
@@ -701,7 +701,7 @@ __device__ int floors_mhd(const koral_params &P, double *pp, const Geom &g)
 
 // check_floors_rad, u2prad.c:687-800
 template <int MAG, int RAD>
-__device__ int floors_rad(const koral_params &P, double *pp, const Geom &g)
+KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
   int ret = 0;
@@ -733,7 +733,7 @@ __device__ int floors_rad(const koral_params &P, double *pp, const Geom &g)
 // calc_primitives core (u2p.c:13-133) for an active, non-axis cell: returns
 // the u2p code; sets flags and updates uu to follow the floored primitives.
 template <int MAG, int RAD>
-__device__ int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
+KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
                                int &fl_entr2, int &fl_hd, int &fl_rad)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};

@@ -22,51 +22,51 @@ struct DGrid {
   int bc[6];
 };
 
-__device__ __forceinline__ long cidx(const DGrid &G, int ix, int iy, int iz)
+KD_INL long cidx(const DGrid &G, int ix, int iy, int iz)
 {
   long y = G.ny > 1 ? iy + G.ngy : 0;
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * G.sy + y) * G.sx + ix + G.ngx;
 }
-__device__ __forceinline__ long fxidx(const DGrid &G, int ix, int iy, int iz)
+KD_INL long fxidx(const DGrid &G, int ix, int iy, int iz)
 {
   long y = G.ny > 1 ? iy + G.ngy : 0;
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * G.sy + y) * (G.sx + 1) + ix + G.ngx;
 }
-__device__ __forceinline__ long fyidx(const DGrid &G, int ix, int iy, int iz)
+KD_INL long fyidx(const DGrid &G, int ix, int iy, int iz)
 {
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * (G.sy + 1) + iy + G.ngy) * G.sx + ix + G.ngx;
 }
-__device__ __forceinline__ long fzidx(const DGrid &G, int ix, int iy, int iz)
+KD_INL long fzidx(const DGrid &G, int ix, int iy, int iz)
 {
   long y = G.ny > 1 ? iy + G.ngy : 0;
   return ((long)(iz + G.ngz) * G.sy + y) * G.sx + ix + G.ngx;
 }
-__device__ __forceinline__ long fidx(const DGrid &G, int d, int ix, int iy, int iz)
+KD_INL long fidx(const DGrid &G, int d, int ix, int iy, int iz)
 {
   return d == 0 ? fxidx(G, ix, iy, iz) : (d == 1 ? fyidx(G, ix, iy, iz) : fzidx(G, ix, iy, iz));
 }
-__device__ __forceinline__ long midx(const DGrid &G, int ix, int iy)
+KD_INL long midx(const DGrid &G, int ix, int iy)
 {
   return (long)(G.ny > 1 ? iy + G.ngy : 0) * G.sx + ix + G.ngx;
 }
-__device__ __forceinline__ int dimn(const DGrid &G, int d) { return d == 0 ? G.nx : (d == 1 ? G.ny : G.nz); }
+KD_INL int dimn(const DGrid &G, int d) { return d == 0 ? G.nx : (d == 1 ? G.ny : G.nz); }
 
-__device__ __forceinline__ double size_x(const DGrid &G, int ic, int d)
+KD_INL double size_x(const DGrid &G, int ic, int d)
 {
   const double *xb = d == 0 ? G.xb0 : (d == 1 ? G.xb1 : G.xb2);
   return xb[ic + 1 + G.ng] - xb[ic + G.ng];
 }
 
-__device__ __forceinline__ void geom_cell(const DGrid &G, int ix, int iy, Geom &g)
+KD_INL void geom_cell(const DGrid &G, int ix, int iy, Geom &g)
 {
   const long m = midx(G, ix, iy), n = G.npl;
   load_geom(g, [&](int c) { return G.gc[c * n + m]; }, [&](int c) { return G.Gc[c * n + m]; });
 }
 
-__device__ __forceinline__ void geom_face(const DGrid &G, int d, int ix, int iy, Geom &g)
+KD_INL void geom_face(const DGrid &G, int d, int ix, int iy, Geom &g)
 {
   if (d == 0) {
     const long m = (long)(G.ny > 1 ? iy + G.ngy : 0) * (G.sx + 1) + ix + G.ngx, n = G.nplx;
@@ -80,9 +80,9 @@ __device__ __forceinline__ void geom_face(const DGrid &G, int d, int ix, int iy,
 }
 
 // ---- reconstruction ----
-__device__ __forceinline__ double mn(double a, double b) { return a < b ? a : b; }
+KD_INL double mn(double a, double b) { return a < b ? a : b; }
 
-__device__ __forceinline__ double minmod3(double a, double b, double c)
+KD_INL double minmod3(double a, double b, double c)
 {
   if (isnan(a) || isnan(b) || isnan(c)) return 1.;
   double x1;
@@ -98,15 +98,15 @@ __device__ __forceinline__ double minmod3(double a, double b, double c)
   return 0.;
 }
 
-__device__ __forceinline__ double DMM(double X, double Y) { return .5 * (sgn(X) + sgn(Y)) * mn(fabs(X), fabs(Y)); }
-__device__ __forceinline__ double DM4(double W, double X, double Y, double Z)
+KD_INL double DMM(double X, double Y) { return .5 * (sgn(X) + sgn(Y)) * mn(fabs(X), fabs(Y)); }
+KD_INL double DM4(double W, double X, double Y, double Z)
 {
   return 0.125 * (sgn(W) + sgn(X)) * fabs((sgn(W) + sgn(Y)) * (sgn(W) + sgn(Z))) *
          mn(mn(fabs(W), fabs(X)), mn(fabs(Y), fabs(Z)));
 }
 
 // MP5 value at the + face of the centre of (um2, um1, u0, up1, up2)
-__device__ __forceinline__ double mp5(double um2, double um1, double u0, double up1, double up2)
+KD_INL double mp5(double um2, double um1, double u0, double up1, double up2)
 {
   const double alpha = 4., epsm = 1.e-10, BB2 = 1.3333333;
   double VOR = (3. * um2 - 20. * um1 + 90. * u0 + 60. * up1 - 5. * up2) / 128.;
@@ -128,7 +128,7 @@ __device__ __forceinline__ double mp5(double um2, double um1, double u0, double 
 
 // PPM (finite.c:142-225) for one variable: returns both face values of the
 // centre cell.  dx[0..4] are the widths of cells -2..+2.
-__device__ __forceinline__ void ppm(const double *v, const double *dx, double &ul, double &ur)
+KD_INL void ppm(const double *v, const double *dx, double &ul, double &ur)
 {
   const double um2 = v[0], um1 = v[1], up1 = v[3], up2 = v[4];
   double u0 = v[2];
@@ -171,7 +171,7 @@ __device__ __forceinline__ void ppm(const double *v, const double *dx, double &u
 
 // ---- physical fluxes ----
 template <int MAG, int RAD>
-__device__ __forceinline__ void flux_prime(const koral_params &P, const double *pp, int d, const Geom &g, double *ff)
+KD_INL void flux_prime(const koral_params &P, const double *pp, int d, const Geom &g, double *ff)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
   double gdetu = g.gdet;
@@ -234,7 +234,7 @@ __device__ __forceinline__ void flux_prime(const koral_params &P, const double *
 // calc_wavespeeds_lr_core (gas: keeps cst1/cst2 order and leaves a0/a1
 // untouched on a negative discriminant) and the radiative variant (min/max,
 // negative discriminant clamped to zero)
-__device__ __forceinline__ void ws_core(const double *ucon, const Geom &g, double w2, int dim, double &a0, double &a1,
+KD_INL void ws_core(const double *ucon, const Geom &g, double w2, int dim, double &a0, double &a1,
                                         bool rad)
 {
   double Acon[4], Bcon[4];
@@ -282,7 +282,7 @@ __device__ __forceinline__ void ws_core(const double *ucon, const Geom &g, doubl
 constexpr int NWS = 18;
 
 template <int MAG, int RAD>
-__device__ __forceinline__ void wavespeeds_cell(const koral_params &P, const DGrid &G, const double *pp, int ix,
+KD_INL void wavespeeds_cell(const koral_params &P, const DGrid &G, const double *pp, int ix,
                                                 int iy, int iz, const Geom &g, double *a)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
@@ -345,7 +345,7 @@ __device__ __forceinline__ void wavespeeds_cell(const koral_params &P, const DGr
 
 // ---- metric source terms (GDETIN=1), physics.c:324-465 ----
 template <int MAG, int RAD>
-__device__ __forceinline__ void metric_source(const koral_params &P, const DGrid &G, const double *pp, int ix, int iy,
+KD_INL void metric_source(const koral_params &P, const DGrid &G, const double *pp, int ix, int iy,
                                               const Geom &g, double *ss)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;

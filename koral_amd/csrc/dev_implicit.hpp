@@ -10,7 +10,7 @@ namespace kd {
 // f_implicit_lab_4dprim: returns -1 on failure, 1 if the M1 inversion was
 // capped, 0 otherwise.  pp0 may be modified (fluid-frame entropy equation).
 template <int MAG, int RAD>
-__device__ int f_implicit(const koral_params &P, const double *ppin, const double *uu0, double *pp0, double dt,
+KD_FUN int f_implicit(const koral_params &P, const double *ppin, const double *uu0, double *pp0, double dt,
                           const Geom &g, double *f, int whichprim, int whicheq, int whichframe, double &err0)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -103,7 +103,7 @@ __device__ int f_implicit(const koral_params &P, const double *ppin, const doubl
 }
 
 // inverse_44matrix (misc.c:502-605): adjugate inverse, fails on NaN entries
-__device__ __forceinline__ int inverse44(const double *a, double *ia)
+KD_INL int inverse44(const double *a, double *ia)
 {
   double s0 = a[0] * a[5] - a[4] * a[1];
   double s1 = a[0] * a[6] - a[4] * a[2];
@@ -149,7 +149,7 @@ __device__ __forceinline__ int inverse44(const double *a, double *ia)
 // Newton-update helper: the "other half" of the state follows from
 // conservation of U_gas+U_rad and an inversion (rad.c:880-950).
 template <int MAG, int RAD>
-__device__ __forceinline__ int complete_state(const koral_params &P, double *pp, double *uu, const double *uu0,
+KD_INL int complete_state(const koral_params &P, double *pp, double *uu, const double *uu0,
                                               const Geom &g, int whichprim, bool final_pass)
 {
   constexpr int EE = Lay<MAG, RAD>::EE;
@@ -184,7 +184,7 @@ __device__ __forceinline__ int complete_state(const koral_params &P, double *pp,
 
 // solve_implicit_lab_4dprim, rad.c:376-1218
 template <int MAG, int RAD>
-__device__ int solve_4dprim(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
+KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
                             int whichprim, int whicheq, int whichframe, double *pp, double global_time, int &iters)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -302,7 +302,7 @@ struct ImpInfo {
 // solve_implicit_lab + implicit_lab_rad_source_term (rad.c:1225-1570,
 // rad.c:2758-2795).  Returns 0 on success with uu/pp updated.
 template <int MAG, int RAD>
-__device__ int solve_implicit_lab(const koral_params &P, double *uu, double *pp, const Geom &g, double dt,
+KD_FUN int solve_implicit_lab(const koral_params &P, double *uu, double *pp, const Geom &g, double dt,
                                   double global_time, ImpInfo &info)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;

@@ -1,0 +1,921 @@
+// Grid-level operators of one KORAL timestep as per-thread functors, and the
+// order in which a step applies them.
+//
+// Each Op* functor does the work of one thread (one cell, face or edge) in
+// operator()(i) for i in [0, size()).  The step order (set_bc, op_explicit,
+// calc_u2p with cell_fixup, op_implicit, RK2IMEX combinations) is written
+// once against an executor Ex that provides
+//   ex.each(op)          run op(i) for every i
+//   ex.max(op, out)      *out = max(*out, max_i op(i))     (op(i) >= 0)
+//   ex.count(op, out3)   out3[k] += sum_i op(i).v[k], k < 3
+// The library's executor launches one HIP kernel per call on the grid's
+// stream (k_grid.hip).
+#pragma once
+#include "dev_grid.hpp"
+
+namespace kd {
+
+struct Box {
+  int x0, y0, z0, nx, ny, nz;
+  KD_HD long size() const { return (long)nx * ny * nz; }
+  KD_HD void at(long i, int &x, int &y, int &z) const
+  {
+    x = x0 + (int)(i % nx);
+    long r = i / nx;
+    y = y0 + (int)(r % ny);
+    z = z0 + (int)(r / ny);
+  }
+};
+
+struct Cnt3 {
+  int v[3];
+};
+
+template <int MAG, int RAD>
+KD_INL void load_p(const double *p, long n, long c, double *v)
+{
+#pragma unroll
+  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) v[iv] = p[iv * n + c];
+}
+template <int MAG, int RAD>
+KD_INL void store_p(double *p, long n, long c, const double *v)
+{
+#pragma unroll
+  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) p[iv * n + c] = v[iv];
+}
+
+template <int MAG, int RAD>
+KD_INL void p2u_at(const koral_params &P, const DGrid &G, const double *p, double *u, int ix, int iy, int iz)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  long c = cidx(G, ix, iy, iz);
+  Geom g;
+  geom_cell(G, ix, iy, g);
+  double pp[NV], uu[NV];
+  load_p<MAG, RAD>(p, G.ncell, c, pp);
+  kd::p2u<MAG, RAD>(P, pp, uu, g);
+  store_p<MAG, RAD>(u, G.ncell, c, uu);
+}
+
+// u = p2u(p) over a box (set_initial_profile leaves u consistent with p)
+template <int MAG, int RAD>
+struct OpP2U {
+  koral_params P;
+  DGrid G;
+  const double *p;
+  double *u;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL void operator()(long i) const
+  {
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    p2u_at<MAG, RAD>(P, G, p, u, ix, iy, iz);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// ghost fill (set_bc finite.c:2752-3700)
+// ---------------------------------------------------------------------------
+// ghost cells outside the domain in exactly one dimension
+template <int MAG, int RAD>
+struct OpBcFaces {
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    int ii[3];
+    b.at(i, ii[0], ii[1], ii[2]);
+    int d = -1, hi = 0, out = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      int n = dimn(G, k);
+      if (n > 1 && ii[k] < 0) { d = k; hi = 0; out++; }
+      if (n > 1 && ii[k] >= n) { d = k; hi = 1; out++; }
+    }
+    if (out != 1) return;
+    int src[3] = {ii[0], ii[1], ii[2]};
+    int n = dimn(G, d);
+    int type = G.bc[2 * d + hi];
+    bool flip = false;
+    if (type == BC_PERIODIC) {
+      if (src[d] < 0) src[d] += n;
+      if (src[d] > n - 1) src[d] -= n;
+    } else if (type == BC_REFLECT) {
+      src[d] = hi ? 2 * n - src[d] - 1 : -src[d] - 1;
+      flip = true;
+    } else {
+      src[d] = hi ? n - 1 : 0;
+    }
+    double pv[NV];
+    load_p<MAG, RAD>(p, G.ncell, cidx(G, src[0], src[1], src[2]), pv);
+    if (flip) {
+      pv[2 + d] = -pv[2 + d];
+      if (MAG) pv[6 + d] = -pv[6 + d];
+      if (RAD) pv[Lay<MAG, RAD>::EE + 1 + d] = -pv[Lay<MAG, RAD>::EE + 1 + d];
+    }
+    long c = cidx(G, ii[0], ii[1], ii[2]);
+    store_p<MAG, RAD>(p, G.ncell, c, pv);
+    Geom g;
+    geom_cell(G, ii[0], ii[1], g);
+    double uu[NV];
+    kd::p2u<MAG, RAD>(P, pv, uu, g);
+    store_p<MAG, RAD>(u, G.ncell, c, uu);
+  }
+};
+
+// Ghosts along the four a-b edges of the domain, one thread per (edge,
+// position along the third dimension c): finite.c:3336-3600 recipe (copies
+// of the far edge ghosts, two diagonal averages) in the reference's order.
+template <int MAG, int RAD>
+struct OpBcEdge {
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  int a, b;
+  KD_HD int nk() const
+  {
+    const int c = 3 - a - b;
+    const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
+    const int ngc = c == 0 ? G.ngx : (c == 1 ? G.ngy : G.ngz);  // 0 if collapsed
+    return nc + 2 * ngc;
+  }
+  KD_HD long size() const { return 4L * nk(); }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    const int NG = G.ng;
+    const int c = 3 - a - b;
+    const int na = dimn(G, a), nb = dimn(G, b);
+    const int ngc = c == 0 ? G.ngx : (c == 1 ? G.ngy : G.ngz);
+    const int n_k = nk();
+    const int corner = (int)(t / n_k);
+    const int k = -ngc + (int)(t % n_k);
+    const int ahi = corner >> 1, bhi = corner & 1;
+    const int ea = ahi ? na : -1, ia = ahi ? na - 1 : 0, ea2 = ahi ? na + 1 : -2;
+    const int eb = bhi ? nb : -1, ib = bhi ? nb - 1 : 0, eb2 = bhi ? nb + 1 : -2;
+    auto cell = [&](int va, int vb) {
+      int q[3];
+      q[a] = va;
+      q[b] = vb;
+      q[c] = k;
+      return cidx(G, q[0], q[1], q[2]);
+    };
+    auto fill_u = [&](int va, int vb) {
+      int q[3];
+      q[a] = va;
+      q[b] = vb;
+      q[c] = k;
+      p2u_at<MAG, RAD>(P, G, p, u, q[0], q[1], q[2]);
+    };
+    const long n = G.ncell;
+    for (int i = 0; i < NG - 1; i++) {
+      int ga = ahi ? na + i + 1 : -NG + i;
+      int gb = bhi ? nb + i + 1 : -NG + i;
+      long d1 = cell(ga, eb), s1 = cell(ga, ib), d2 = cell(ea, gb), s2 = cell(ia, gb);
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) p[iv * n + d1] = p[iv * n + s1];
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) p[iv * n + d2] = p[iv * n + s2];
+      fill_u(ga, eb);
+      fill_u(ea, gb);
+    }
+    {
+      long dd = cell(ea, eb), aa = cell(ea, ib), bb = cell(ia, eb);
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) p[iv * n + dd] = .5 * (p[iv * n + aa] + p[iv * n + bb]);
+      fill_u(ea, eb);
+    }
+    {
+      long dd = cell(ea2, eb2), aa = cell(ea2, eb), bb = cell(ea, eb2);
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) p[iv * n + dd] = .5 * (p[iv * n + aa] + p[iv * n + bb]);
+      fill_u(ea2, eb2);
+    }
+  }
+};
+
+// the eight NG^3 blocks at the domain corners: copies of the corner cell
+template <int MAG, int RAD>
+struct OpBcTotal {
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  KD_HD long size() const { return 8L * G.ng * G.ng * G.ng; }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    const int NG = G.ng;
+    const long per = (long)NG * NG * NG;
+    int blk = (int)(t / per);
+    int r = (int)(t % per);
+    int hx = blk & 1, hy = (blk >> 1) & 1, hz = (blk >> 2) & 1;
+    int ix = (hx ? G.nx : -NG) + r % NG;
+    int iy = (hy ? G.ny : -NG) + (r / NG) % NG;
+    int iz = (hz ? G.nz : -NG) + r / (NG * NG);
+    long s = cidx(G, hx ? G.nx - 1 : 0, hy ? G.ny - 1 : 0, hz ? G.nz - 1 : 0);
+    long d = cidx(G, ix, iy, iz);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) p[iv * G.ncell + d] = p[iv * G.ncell + s];
+    p2u_at<MAG, RAD>(P, G, p, u, ix, iy, iz);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// wave speeds on loop_1 (domain +- lim) and the time-step density
+// (calc_wavespeeds finite.c:568-599, save_wavespeeds finite.c:296-398)
+// ---------------------------------------------------------------------------
+template <int MAG, int RAD>
+struct OpWavespeeds {
+  koral_params P;
+  DGrid G;
+  const double *p;
+  double *ws;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL double operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    int out = (ix < 0 || ix >= G.nx) + (iy < 0 || iy >= G.ny) + (iz < 0 || iz >= G.nz);
+    if (!(MAG || out <= 1)) return 0.;
+    long c = cidx(G, ix, iy, iz);
+    double pp[NV], a[NWS];
+    load_p<MAG, RAD>(p, G.ncell, c, pp);
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    wavespeeds_cell<MAG, RAD>(P, G, pp, ix, iy, iz, g, a);
+#pragma unroll
+    for (int k = 0; k < NWS; k++) ws[k * G.ncell + c] = a[k];
+    if (out != 0) return 0.;
+    double w[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) w[d] = RAD ? mx(a[12 + d], a[15 + d]) : a[12 + d];
+    double dx = size_x(G, ix, 0), dy = size_x(G, iy, 1), dz = size_x(G, iz, 2);
+    double t;
+    if (G.nz > 1 && G.ny > 1)
+      t = w[0] / dx + w[1] / dy + w[2] / dz;
+    else if (G.nz == 1 && G.ny > 1)
+      t = w[0] / dx + w[1] / dy;
+    else if (G.ny == 1 && G.nz > 1)
+      t = w[0] / dx + w[2] / dz;
+    else
+      t = w[0] / dx;
+    return t / P.tsteplim;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// per face: reconstruct both sides, face floors, fluxes, Riemann solver
+// (finite.c:694-1014 and f_calc_fluxes_at_faces finite.c:1297-1415)
+// ---------------------------------------------------------------------------
+template <int MAG, int RAD, int ORDER>
+KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__restrict__ p, int d, int ix, int iy,
+                       int iz, double *pL, double *pR)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  const long n = G.ncell;
+  if (ORDER == 1) {
+    long c[4];
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+      int q[3] = {ix, iy, iz};
+      q[d] += o - 2;
+      c[o] = cidx(G, q[0], q[1], q[2]);
+    }
+    const double th = P.minmod_theta;
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) {
+      double um1 = p[iv * n + c[0]], u0 = p[iv * n + c[1]], up1 = p[iv * n + c[2]];
+      pL[iv] = u0 + .5 * minmod3(th * (up1 - u0), .5 * (up1 - um1), th * (u0 - um1));
+      double vm1 = u0, v0 = up1, vp1 = p[iv * n + c[3]];
+      pR[iv] = v0 - .5 * minmod3(th * (vp1 - v0), .5 * (vp1 - vm1), th * (v0 - vm1));
+    }
+    return;
+  }
+  // cells -3..+2 relative to the face (face between cells -1 and 0)
+  long c[6];
+  double dx[6];
+#pragma unroll
+  for (int o = 0; o < 6; o++) {
+    int q[3] = {ix, iy, iz};
+    q[d] += o - 3;
+    c[o] = cidx(G, q[0], q[1], q[2]);
+    dx[o] = (ORDER == 2) ? size_x(G, q[d], d) : 0.;
+  }
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) {
+    if (ORDER == 4) {
+      double v0 = p[iv * n + c[0]], v1 = p[iv * n + c[1]], v2 = p[iv * n + c[2]], v3 = p[iv * n + c[3]],
+             v4 = p[iv * n + c[4]], v5 = p[iv * n + c[5]];
+      pL[iv] = mp5(v0, v1, v2, v3, v4);  // right face of cell -1
+      pR[iv] = mp5(v5, v4, v3, v2, v1);  // left face of cell 0
+    } else {
+      double v[6];
+#pragma unroll
+      for (int o = 0; o < 6; o++) v[o] = p[iv * n + c[o]];
+      double ul, ur;
+      ppm(v, dx, ul, ur);
+      pL[iv] = ur;
+      ppm(v + 1, dx + 1, ul, ur);
+      pR[iv] = ul;
+    }
+  }
+}
+
+template <int MAG, int RAD, int ORDER>
+struct OpFlux {
+  koral_params P;
+  DGrid G;
+  const double *p, *ws;
+  double *flb;
+  int d;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    constexpr int NVMHD = 6 + 3 * MAG;
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    double pL[NV], pR[NV];
+    recon_face<MAG, RAD, ORDER>(P, G, p, d, ix, iy, iz, pL, pR);
+    Geom g;
+    geom_face(G, d, ix, iy, g);
+    floors_mhd<MAG, RAD>(P, pL, g);
+    floors_mhd<MAG, RAD>(P, pR, g);
+    double fL[NV], fR[NV], uL[NV], uR[NV];
+    flux_prime<MAG, RAD>(P, pL, d + 1, g, fL);
+    flux_prime<MAG, RAD>(P, pR, d + 1, g, fR);
+    kd::p2u<MAG, RAD>(P, pL, uL, g);
+    kd::p2u<MAG, RAD>(P, pR, uR, g);
+    int qm[3] = {ix, iy, iz};
+    qm[d] -= 1;
+    const long cR = cidx(G, ix, iy, iz), cL = cidx(G, qm[0], qm[1], qm[2]);
+    const long n = G.ncell;
+    const long f = fidx(G, d, ix, iy, iz);
+    const long nf = d == 0 ? G.nfx : (d == 1 ? G.nfy : G.nfz);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) {
+      const int r = (RAD && iv >= NVMHD) ? 1 : 0;
+      double ag = mx(ws[(12 + 3 * r + d) * n + cR], ws[(12 + 3 * r + d) * n + cL]);
+      double al = mn(ws[(6 * r + 2 * d) * n + cR], ws[(6 * r + 2 * d) * n + cL]);
+      double ar = mx(ws[(6 * r + 2 * d + 1) * n + cR], ws[(6 * r + 2 * d + 1) * n + cL]);
+      double F;
+      if (P.fluxmethod == 0) {
+        F = .5 * (fR[iv] + fL[iv] - ag * (uR[iv] - uL[iv]));
+      } else {
+        if (al > 0.)
+          F = fL[iv];
+        else if (ar < 0.)
+          F = fL[iv];
+        else
+          F = (-al * fR[iv] + ar * fL[iv] + al * ar * (uR[iv] - uL[iv])) / (ar - al);
+      }
+      flb[iv * nf + f] = F;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// flux-CT (magn.c:199-339): corner EMFs, then the B fluxes of every face
+// ---------------------------------------------------------------------------
+struct Flx {
+  double *x, *y, *z;
+  long nfx, nfy, nfz;
+};
+
+struct OpEmf {
+  DGrid G;
+  Flx F;
+  double *emf;
+  KD_HD long size() const { return (long)(G.nx + 1) * (G.ny + 1) * (G.nz + 1); }
+  KD_INL void operator()(long t) const
+  {
+    const int NX = G.nx, NY = G.ny, NZ = G.nz;
+    const long ne = size();
+    int ix = (int)(t % (NX + 1));
+    int iy = (int)((t / (NX + 1)) % (NY + 1));
+    int iz = (int)(t / ((long)(NX + 1) * (NY + 1)));
+    const double c1 = (NY > 1 && NZ > 1) ? 0.25 : 0.5;
+    const double c2 = (NX > 1 && NZ > 1) ? 0.25 : 0.5;
+    const double c3 = (NX > 1 && NY > 1) ? 0.25 : 0.5;
+    auto FX = [&](int iv, int a, int b, int c) { return F.x[iv * F.nfx + fxidx(G, a, NY > 1 ? b : 0, NZ > 1 ? c : 0)]; };
+    auto FY = [&](int iv, int a, int b, int c) { return F.y[iv * F.nfy + fyidx(G, NX > 1 ? a : 0, b, NZ > 1 ? c : 0)]; };
+    auto FZ = [&](int iv, int a, int b, int c) { return F.z[iv * F.nfz + fzidx(G, NX > 1 ? a : 0, NY > 1 ? b : 0, c)]; };
+    double e1 = 0., e2 = 0., e3 = 0.;
+    if (NY > 1 || NZ > 1) {
+      double s = 0.;
+      if (NY > 1) s = s + FY(8, ix, iy, iz) + FY(8, ix, iy, iz - 1);
+      if (NZ > 1) s = s - FZ(7, ix, iy, iz) - FZ(7, ix, iy - 1, iz);
+      e1 = c1 * (s);
+    }
+    if (NX > 1 || NZ > 1) {
+      double s = 0.;
+      if (NZ > 1) s = s + FZ(6, ix, iy, iz) + FZ(6, ix - 1, iy, iz);
+      if (NX > 1) s = s - FX(8, ix, iy, iz) - FX(8, ix, iy, iz - 1);
+      e2 = c2 * (s);
+    }
+    if (NX > 1 || NY > 1) {
+      double s = 0.;
+      if (NX > 1) s = s + FX(7, ix, iy, iz) + FX(7, ix, iy - 1, iz);
+      if (NY > 1) s = s - FY(6, ix, iy, iz) - FY(6, ix - 1, iy, iz);
+      e3 = c3 * (s);
+    }
+    emf[0 * ne + t] = e1;
+    emf[1 * ne + t] = e2;
+    emf[2 * ne + t] = e3;
+  }
+};
+
+struct OpCtFaces {
+  DGrid G;
+  Flx F;
+  const double *emf;
+  KD_HD long size() const { return (long)(G.nx + 1) * (G.ny + 1) * (G.nz + 1); }
+  KD_INL void operator()(long t) const
+  {
+    const int NX = G.nx, NY = G.ny, NZ = G.nz;
+    const long ne = size();
+    int ix = (int)(t % (NX + 1));
+    int iy = (int)((t / (NX + 1)) % (NY + 1));
+    int iz = (int)(t / ((long)(NX + 1) * (NY + 1)));
+    auto E = [&](int k, int a, int b, int c) {
+      return emf[(k - 1) * ne + a + (long)b * (NX + 1) + (long)c * (NY + 1) * (NX + 1)];
+    };
+    if (NX > 1 && iy < NY && iz < NZ) {
+      long f = fxidx(G, ix, iy, iz);
+      F.x[6 * F.nfx + f] = 0.;
+      F.x[7 * F.nfx + f] = 0.5 * (E(3, ix, iy, iz) + E(3, ix, iy + 1, iz));
+      F.x[8 * F.nfx + f] = -0.5 * (E(2, ix, iy, iz) + E(2, ix, iy, iz + 1));
+    }
+    if (NY > 1 && ix < NX && iz < NZ) {
+      long f = fyidx(G, ix, iy, iz);
+      F.y[6 * F.nfy + f] = -0.5 * (E(3, ix, iy, iz) + E(3, ix + 1, iy, iz));
+      F.y[7 * F.nfy + f] = 0.;
+      F.y[8 * F.nfy + f] = 0.5 * (E(1, ix, iy, iz) + E(1, ix, iy, iz + 1));
+    }
+    if (NZ > 1 && ix < NX && iy < NY) {
+      long f = fzidx(G, ix, iy, iz);
+      F.z[6 * F.nfz + f] = 0.5 * (E(2, ix, iy, iz) + E(2, ix + 1, iy, iz));
+      F.z[7 * F.nfz + f] = -0.5 * (E(1, ix, iy, iz) + E(1, ix, iy + 1, iz));
+      F.z[8 * F.nfz + f] = 0.;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// conservative update with metric sources (finite.c:1062-1160)
+// ---------------------------------------------------------------------------
+template <int MAG, int RAD>
+struct OpUpdate {
+  koral_params P;
+  DGrid G;
+  const double *p;
+  double *u;
+  Flx F;
+  double dt;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    Box b{0, 0, 0, G.nx, G.ny, G.nz};
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    long c = cidx(G, ix, iy, iz);
+    double pp[NV], ms[NV];
+    load_p<MAG, RAD>(p, G.ncell, c, pp);
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    metric_source<MAG, RAD>(P, G, pp, ix, iy, g, ms);
+    double dx = size_x(G, ix, 0), dy = size_x(G, iy, 1), dz = size_x(G, iz, 2);
+    long fxl = fxidx(G, ix, iy, iz), fxr = fxidx(G, ix + 1, iy, iz);
+    long fyl = 0, fyr = 0, fzl = 0, fzr = 0;
+    if (G.ny > 1) { fyl = fyidx(G, ix, iy, iz); fyr = fyidx(G, ix, iy + 1, iz); }
+    if (G.nz > 1) { fzl = fzidx(G, ix, iy, iz); fzr = fzidx(G, ix, iy, iz + 1); }
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) {
+      double du = -(F.x[iv * F.nfx + fxr] - F.x[iv * F.nfx + fxl]) * dt / dx;
+      if (G.ny > 1) du = du - (F.y[iv * F.nfy + fyr] - F.y[iv * F.nfy + fyl]) * dt / dy;
+      if (G.nz > 1) du = du - (F.z[iv * F.nfz + fzr] - F.z[iv * F.nfz + fzl]) * dt / dz;
+      u[iv * G.ncell + c] = u[iv * G.ncell + c] + du + ms[iv] * dt;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// u2p sweep (calc_u2p finite.c:502-561 / calc_primitives u2p.c:13-133);
+// counts {hd fixup requests, rad fixup requests, entropy inversions}
+// ---------------------------------------------------------------------------
+template <int MAG, int RAD>
+struct OpU2P {
+  koral_params P;
+  DGrid G;
+  double *u, *p;
+  signed char *fl;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL Cnt3 operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    Box b{0, 0, 0, G.nx, G.ny, G.nz};
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    long c = cidx(G, ix, iy, iz);
+    double pp[NV], uu[NV];
+    load_p<MAG, RAD>(p, G.ncell, c, pp);
+    load_p<MAG, RAD>(u, G.ncell, c, uu);
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    int fe, fe2, fh, fr;
+    calc_primitives<MAG, RAD>(P, uu, pp, g, fe, fe2, fh, fr);
+    store_p<MAG, RAD>(p, G.ncell, c, pp);
+    store_p<MAG, RAD>(u, G.ncell, c, uu);
+    const long n = G.ncell;
+    fl[F_ENTROPY * n + c] = (signed char)fe;
+    fl[F_ENTROPY2 * n + c] = (signed char)fe2;
+    fl[F_HDFIXUP * n + c] = (signed char)fh;
+    fl[F_RADFIXUP * n + c] = (signed char)fr;
+    return Cnt3{{fh != 0, fr != 0, fe != 0}};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// cell_fixup (finite.c:4468-4640): the reference processes flagged cells in
+// (ix, iy, iz) order and clears each flag when processed, so a lower
+// neighbour always counts as good and an upper neighbour only if it was not
+// flagged.  Results go to scratch (pn, un, mark) and are applied by
+// OpFixupApply, so every cell reads its neighbours' values from before the
+// sweep.
+// ---------------------------------------------------------------------------
+template <int MAG, int RAD>
+struct OpFixup {
+  koral_params P;
+  DGrid G;
+  const double *p;
+  const signed char *fl;
+  int type;
+  double *pn, *un;
+  int *mark;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    constexpr int EE = Lay<MAG, RAD>::EE;
+    Box b{0, 0, 0, G.nx, G.ny, G.nz};
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    const long n = G.ncell;
+    long c = cidx(G, ix, iy, iz);
+    mark[c] = 0;
+    if (fl[type * n + c] == 0) return;
+    const int NX = G.nx, NY = G.ny, NZ = G.nz;
+    const int nb[6][3] = {{ix - 1, iy, iz}, {ix + 1, iy, iz}, {ix, iy - 1, iz},
+                          {ix, iy + 1, iz}, {ix, iy, iz - 1}, {ix, iy, iz + 1}};
+    const bool inb[6] = {ix - 1 >= 0, ix + 1 < NX, iy - 1 >= 0, iy + 1 < NY, iz - 1 >= 0, iz + 1 < NZ};
+    long cn[6];
+    int in = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      if (!inb[k]) continue;
+      long q = cidx(G, nb[k][0], nb[k][1], nb[k][2]);
+      bool lower = (k & 1) == 0;
+      if (lower || fl[type * n + q] == 0) cn[in++] = q;
+    }
+    if ((NZ == 1 && NY == 1 && in >= 1) || (NZ == 1 && in >= 2) || (NY == 1 && in >= 2) || in >= 3) {
+      double pp[NV], uu[NV];
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) {
+        bool fix = false;
+        if (type == F_HDFIXUP && iv != 0 && iv < 6) fix = true;
+        if (type == F_RADFIXUP && iv != 0 && RAD && iv >= EE && iv <= EE + 3) fix = true;
+        if (type == F_RADIMPFIXUP && iv != 0 && (iv < 6 || (RAD && iv >= EE && iv <= EE + 3))) fix = true;
+        if (fix) {
+          double s = 0.;
+          for (int q = 0; q < in; q++) s += p[iv * n + cn[q]];
+          pp[iv] = s / (double)in;
+        } else {
+          pp[iv] = p[iv * n + c];
+        }
+      }
+      Geom g;
+      geom_cell(G, ix, iy, g);
+      kd::p2u<MAG, RAD>(P, pp, uu, g);
+#pragma unroll
+      for (int iv = 0; iv < 6 + 3 * MAG; iv++) {
+        pn[iv * n + c] = pp[iv];
+        un[iv * n + c] = uu[iv];
+      }
+      mark[c] = 1;
+    }
+  }
+};
+
+template <int MAG>
+struct OpFixupApply {
+  DGrid G;
+  double *p, *u;
+  signed char *fl;
+  int type;
+  const double *pn, *un;
+  const int *mark;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL void operator()(long i) const
+  {
+    Box b{0, 0, 0, G.nx, G.ny, G.nz};
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    const long n = G.ncell;
+    long c = cidx(G, ix, iy, iz);
+    fl[type * n + c] = 0;
+    if (!mark[c]) return;
+#pragma unroll
+    for (int iv = 0; iv < 6 + 3 * MAG; iv++) {
+      p[iv * n + c] = pn[iv * n + c];
+      u[iv * n + c] = un[iv * n + c];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// implicit sweep (op_implicit finite.c:1242-1292);
+// counts {successes, Newton iterations of the successful attempts, failures}
+// ---------------------------------------------------------------------------
+template <int MAG>
+struct OpImplicit {
+  koral_params P;
+  DGrid G;
+  double *u, *p;
+  signed char *fl;
+  double dt, t;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL Cnt3 operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, 1>::NV;
+    Box b{0, 0, 0, G.nx, G.ny, G.nz};
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    long c = cidx(G, ix, iy, iz);
+    double pp[NV], uu[NV];
+    load_p<MAG, 1>(p, G.ncell, c, pp);
+    load_p<MAG, 1>(u, G.ncell, c, uu);
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    ImpInfo inf;
+    int r = solve_implicit_lab<MAG, 1>(P, uu, pp, g, dt, t, inf);
+    if (r == 0) {
+      store_p<MAG, 1>(p, G.ncell, c, pp);
+      store_p<MAG, 1>(u, G.ncell, c, uu);
+    }
+    const long n = G.ncell;
+    fl[F_RADSOURCETYPE * n + c] = 3;
+    fl[F_RADFIXUP * n + c] = r == 0 ? 0 : -1;
+    fl[F_RADIMPFIXUP * n + c] = r == 0 ? 0 : -1;
+    return Cnt3{{r == 0, inf.iters, r != 0}};
+  }
+};
+
+// RK2IMEX linear combination over the domain: out = a x + b y (+ c z)
+struct OpComb {
+  DGrid G;
+  int nv;
+  double *out;
+  double a, b, c;
+  const double *x, *y, *z;
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz * nv; }
+  KD_INL void operator()(long i) const
+  {
+    long nd = (long)G.nx * G.ny * G.nz;
+    int iv = (int)(i / nd);
+    long r = i % nd;
+    int ix = (int)(r % G.nx), iy = (int)((r / G.nx) % G.ny), iz = (int)(r / ((long)G.nx * G.ny));
+    long k = (long)iv * G.ncell + cidx(G, ix, iy, iz);
+    double v;
+    if (z)
+      v = a * x[k] + b * y[k] + c * z[k];
+    else if (y)
+      v = a * x[k] + b * y[k];
+    else
+      v = a * x[k];
+    out[k] = v;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// step order, written once for any executor
+// ---------------------------------------------------------------------------
+// Device-resident state of a grid: SoA primitives/conserved over the
+// ghost-inclusive box, int8 flags, and the work arrays of a step.
+struct GridBufs {
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  signed char *fl;
+  double *ws, *flb[3], *emf;
+  double *pn, *un;
+  int *mark;
+  double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
+  double *tsd;    // time-step density max (one double)
+  int *counters;  // step counters [6]
+};
+
+// Ghost-inclusive layout of a grid description (a collapsed dimension,
+// n = 1, has no ghosts).  Table pointers are left for the caller.
+inline void grid_layout(const koral_grid_desc *D, DGrid &G)
+{
+  G.nx = D->nx; G.ny = D->ny; G.nz = D->nz; G.ng = D->ng;
+  G.ngx = D->ng; G.ngy = D->ny > 1 ? D->ng : 0; G.ngz = D->nz > 1 ? D->ng : 0;
+  G.sx = G.nx + 2 * G.ngx; G.sy = G.ny + 2 * G.ngy; G.sz = G.nz + 2 * G.ngz;
+  G.ncell = (long)G.sx * G.sy * G.sz;
+  G.nfx = (long)(G.sx + 1) * G.sy * G.sz;
+  G.nfy = (long)G.sx * (G.sy + 1) * G.sz;
+  G.nfz = (long)G.sx * G.sy * (G.sz + 1);
+  G.npl = (long)G.sy * G.sx;
+  G.nplx = (long)G.sy * (G.sx + 1);
+  G.nply = (long)(G.sy + 1) * G.sx;
+  for (int i = 0; i < 6; i++) G.bc[i] = D->bc[i];
+}
+
+template <int M, int R>
+struct Tag {
+  static constexpr int MAG = M, RAD = R;
+};
+
+template <class F>
+void dispatch(const koral_params &P, F f)
+{
+  if (P.mag && P.rad)
+    f(Tag<1, 1>{});
+  else if (P.mag)
+    f(Tag<1, 0>{});
+  else if (P.rad)
+    f(Tag<0, 1>{});
+  else
+    f(Tag<0, 0>{});
+}
+
+inline Box wavespeed_box(const koral_params &P, const DGrid &G)
+{
+  int lim = (P.int_order == 4) ? 2 : 1;
+  int lx = lim, ly = G.ny > 1 ? lim : 0, lz = G.nz > 1 ? lim : 0;
+  return Box{-lx, -ly, -lz, G.nx + 2 * lx, G.ny + 2 * ly, G.nz + 2 * lz};
+}
+
+template <class Ex>
+void grid_p2u(Ex &ex, const GridBufs &s)
+{
+  const DGrid &G = s.G;
+  Box b{-G.ngx, -G.ngy, -G.ngz, G.sx, G.sy, G.sz};
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    ex.each(OpP2U<T::MAG, T::RAD>{s.P, G, s.p, s.u, b});
+  });
+}
+
+template <class Ex>
+void set_bc(Ex &ex, const GridBufs &s)
+{
+  const DGrid &G = s.G;
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    constexpr int M = T::MAG, R = T::RAD;
+    ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u, Box{-G.ngx, -G.ngy, -G.ngz, G.sx, G.sy, G.sz}});
+    if (G.ny > 1) {
+      ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 0, 1});
+      if (G.nz > 1) {
+        ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 0, 2});
+        ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 1, 2});
+        ex.each(OpBcTotal<M, R>{s.P, G, s.p, s.u});
+      }
+    }
+  });
+}
+
+template <class Ex>
+void tstepden(Ex &ex, const GridBufs &s)
+{
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    ex.max(OpWavespeeds<T::MAG, T::RAD>{s.P, s.G, s.p, s.ws, wavespeed_box(s.P, s.G)}, s.tsd);
+  });
+}
+
+// op_explicit: ghosts, wave speeds (tsd accumulates the max), face fluxes,
+// flux-CT, update
+template <class Ex>
+void explicit_op(Ex &ex, const GridBufs &s, double dt)
+{
+  const DGrid &G = s.G;
+  const koral_params &P = s.P;
+  set_bc(ex, s);
+  tstepden(ex, s);
+  const int tt = P.mag ? 1 : 0;
+  const int n[3] = {G.nx, G.ny, G.nz};
+  for (int d = 0; d < 3; d++) {
+    if (n[d] == 1) continue;
+    int lo[3], cnt[3];
+    for (int k = 0; k < 3; k++) {
+      if (k == d) { lo[k] = 0; cnt[k] = n[k] + 1; }
+      else if (n[k] == 1) { lo[k] = 0; cnt[k] = 1; }
+      else { lo[k] = -tt; cnt[k] = n[k] + 2 * tt; }
+    }
+    Box fb{lo[0], lo[1], lo[2], cnt[0], cnt[1], cnt[2]};
+    dispatch(P, [&](auto t) {
+      using T = decltype(t);
+      constexpr int M = T::MAG, R = T::RAD;
+      if (P.int_order == 4)
+        ex.each(OpFlux<M, R, 4>{P, G, s.p, s.ws, s.flb[d], d, fb});
+      else if (P.int_order == 2)
+        ex.each(OpFlux<M, R, 2>{P, G, s.p, s.ws, s.flb[d], d, fb});
+      else
+        ex.each(OpFlux<M, R, 1>{P, G, s.p, s.ws, s.flb[d], d, fb});
+    });
+  }
+  Flx F{s.flb[0], s.flb[1], s.flb[2], G.nfx, G.nfy, G.nfz};
+  if (P.mag) {
+    ex.each(OpEmf{G, F, s.emf});
+    ex.each(OpCtFaces{G, F, s.emf});
+  }
+  dispatch(P, [&](auto t) {
+    using T = decltype(t);
+    ex.each(OpUpdate<T::MAG, T::RAD>{P, G, s.p, s.u, F, dt});
+  });
+}
+
+template <class Ex>
+void fixup(Ex &ex, const GridBufs &s, int type)
+{
+  const koral_params &P = s.P;
+  if (!P.dofixups) return;
+  if (type == F_HDFIXUP && !P.dou2pmhdfixups) return;
+  if (type == F_RADFIXUP && !P.dou2pradfixups) return;
+  if (type == F_RADIMPFIXUP && !P.doradimpfixups) return;
+  dispatch(P, [&](auto t) {
+    using T = decltype(t);
+    ex.each(OpFixup<T::MAG, T::RAD>{P, s.G, s.p, s.fl, type, s.pn, s.un, s.mark});
+    ex.each(OpFixupApply<T::MAG>{s.G, s.p, s.u, s.fl, type, s.pn, s.un, s.mark});
+  });
+}
+
+// calc_u2p with its cell_fixup passes and the ghost refresh
+template <class Ex>
+void u2p(Ex &ex, const GridBufs &s)
+{
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    ex.count(OpU2P<T::MAG, T::RAD>{s.P, s.G, s.u, s.p, s.fl}, s.counters + 3);
+  });
+  fixup(ex, s, F_HDFIXUP);
+  if (s.P.rad) fixup(ex, s, F_RADFIXUP);
+  set_bc(ex, s);
+}
+
+template <class Ex>
+void implicit_op(Ex &ex, const GridBufs &s, double t, double dt)
+{
+  if (!s.P.rad) return;
+  if (s.P.mag)
+    ex.count(OpImplicit<1>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+  else
+    ex.count(OpImplicit<0>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+  fixup(ex, s, F_RADIMPFIXUP);
+}
+
+template <class Ex>
+void comb(Ex &ex, const GridBufs &s, double *out, double a, const double *x, double b = 0.,
+          const double *y = nullptr, double c = 0., const double *z = nullptr)
+{
+  ex.each(OpComb{s.G, s.P.nv, out, a, b, c, x, y, z});
+}
+
+// body of one RK2IMEX step, ko.c:444-570
+template <class Ex>
+void step_rk2imex(Ex &ex, const GridBufs &s, double t, double dt)
+{
+  const double gamma = 1. - 1. / sqrt(2.);
+  comb(ex, s, s.ut0, 1., s.u);
+  implicit_op(ex, s, t, dt * gamma);
+  comb(ex, s, s.drt1, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.ut0);
+  comb(ex, s, s.ut1, 1., s.u);
+  u2p(ex, s);
+  explicit_op(ex, s, dt);
+  comb(ex, s, s.dut1, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
+  comb(ex, s, s.u, 1., s.ut0, (dt), s.dut1, (dt * (1. - 2. * gamma)), s.drt1);
+  comb(ex, s, s.uforget, 1., s.u);
+  u2p(ex, s);
+  implicit_op(ex, s, t, gamma * dt);
+  comb(ex, s, s.drt2, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.uforget);
+  comb(ex, s, s.ut1, 1., s.u);  // ut2
+  u2p(ex, s);
+  explicit_op(ex, s, dt);
+  comb(ex, s, s.dut2, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
+  comb(ex, s, s.u, 1., s.ut0, (dt / 2.), s.dut1, (dt / 2.), s.dut2);
+  comb(ex, s, s.u, 1., s.u, (dt / 2.), s.drt1, (dt / 2.), s.drt2);
+  u2p(ex, s);
+}
+
+}  // namespace kd

@@ -34,6 +34,7 @@ def _sig(L):
         "koral_hip_grid_upload": [_vp, _vp, _vp, _vp],
         "koral_hip_grid_download": [_vp, _vp, _vp, _vp],
         "koral_hip_grid_device_state": [_vp, _vp, _vp, _vp],
+        "koral_hip_grid_download_work": [_vp, i, _vp],
         "koral_hip_grid_p2u": [_vp],
         "koral_hip_set_bc": [_vp],
         "koral_hip_tstepden": [_vp, _vp],
@@ -105,6 +106,26 @@ class DeviceGrid:
         check(self.L.koral_hip_grid_download(self.h, p.ctypes.data_as(_vp), u.ctypes.data_as(_vp),
                                              f.ctypes.data_as(_vp)), "grid_download")
         return p, u, f
+
+    def work(self):
+        """Wave speeds [z, y, x, 18] and face fluxes after flux-CT per
+        direction ([z, y, x+1, nv], [z, y+1, x, nv], [z+1, y, x, nv]; None for
+        a collapsed direction) of the last op_explicit."""
+        s = self.spec
+        sx, sy, sz, nv = s.sx, s.sy, s.sz, self.P.nv
+        ws = np.zeros((18, sz, sy, sx))
+        check(self.L.koral_hip_grid_download_work(self.h, 0, ws.ctypes.data_as(_vp)), "download_work")
+        shapes = [(nv, sz, sy, sx + 1), (nv, sz, sy + 1, sx), (nv, sz + 1, sy, sx)]
+        n = [s.nx, s.ny, s.nz]
+        fl = []
+        for d in range(3):
+            if n[d] == 1:
+                fl.append(None)
+                continue
+            f = np.zeros(shapes[d])
+            check(self.L.koral_hip_grid_download_work(self.h, d + 1, f.ctypes.data_as(_vp)), "download_work")
+            fl.append(np.ascontiguousarray(np.moveaxis(f, 0, -1)))
+        return np.ascontiguousarray(np.moveaxis(ws, 0, -1)), fl
 
     def p2u(self):
         check(self.L.koral_hip_grid_p2u(self.h), "grid_p2u")

@@ -158,6 +158,13 @@ int oracle_grid_explicit(const oracle_params *P, const o_grid *G, double t, doub
   return o_op_explicit(P, G, t, dt, u, p, tstepdenmax);
 }
 
+int oracle_grid_explicit_parts(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                               double *tstepdenmax, double *ws, double *flx, double *fly, double *flz)
+{
+  double *fl[3] = {flx, fly, flz};
+  return o_op_explicit_parts(P, G, t, dt, u, p, tstepdenmax, ws, fl);
+}
+
 int oracle_grid_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts)
 {
   return o_calc_u2p(P, G, u, p, flags, counts);

@@ -761,6 +761,15 @@ static void metric_source(const oracle_params *P, const o_grid *G, const double 
 int o_op_explicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
                   double *tstepdenmax)
 {
+  return o_op_explicit_parts(P, G, t, dt, u, p, tstepdenmax, NULL, NULL);
+}
+
+/* op_explicit that also hands out its intermediate arrays: the wave speeds
+   [cell][NWS] of the sweep and the face fluxes [face][nv] per direction
+   after flux-CT (either pointer may be NULL). */
+int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                        double *tstepdenmax, double *ws_out, double **flb_out)
+{
   int nv = P->nv;
   (void)t;
   o_set_bc(P, G, u, p);
@@ -799,6 +808,10 @@ int o_op_explicit(const oracle_params *P, const o_grid *G, double t, double dt, 
           u[(long)nv * c + iv] = u[(long)nv * c + iv] + du + ms[iv] * dt;
         }
       }
+  if (ws_out) memcpy(ws_out, ws, sizeof(double) * ncell * NWS);
+  if (flb_out)
+    for (int d = 0; d < 3; d++)
+      if (flb[d] && flb_out[d]) memcpy(flb_out[d], flb[d], sizeof(double) * nf[d] * nv);
   for (int d = 0; d < 3; d++) free(flb[d]);
   free(ws);
   return 0;

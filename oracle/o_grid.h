@@ -23,6 +23,8 @@ void o_geom_face(const o_grid *G, int d, int ix, int iy, o_geom *g);
 void o_set_bc(const oracle_params *P, const o_grid *G, double *u, double *p);
 int o_op_explicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
                   double *tstepdenmax);
+int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                        double *tstepdenmax, double *ws_out, double **flb_out);
 int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts);
 int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                   int *counts);

@@ -188,6 +188,8 @@ def _grid_lib():
         L.oracle_grid_tstepden.argtypes = [vp, vp, _dp]
         L.oracle_grid_tstepden.restype = ctypes.c_double
         L.oracle_grid_explicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_grid_explicit_parts.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                                 _dp, _dp, _dp]
         L.oracle_grid_u2p.argtypes = [vp, vp, _dp, _dp, _ip, _ip]
         L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
         L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
@@ -222,6 +224,22 @@ class GridState:
         _grid_lib().oracle_grid_explicit(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
                                          _d(tm))
         return tm[0]
+
+    def explicit_parts(self, t, dt):
+        """op_explicit that also returns the wave speeds [z, y, x, 18] and the
+        face fluxes after flux-CT per direction ([z, y, x+1, nv],
+        [z, y+1, x, nv], [z+1, y, x, nv]; None for a collapsed direction)."""
+        sp = self.grid.spec
+        sx, sy, sz, nv = sp.sx, sp.sy, sp.sz, self.P.nv
+        ws = np.zeros((sz, sy, sx, 18))
+        shapes = [(sz, sy, sx + 1, nv), (sz, sy + 1, sx, nv), (sz + 1, sy, sx, nv)]
+        n = [sp.nx, sp.ny, sp.nz]
+        fl = [np.zeros(shapes[d]) if n[d] > 1 else None for d in range(3)]
+        tm = np.array([-1.0])
+        ptr = [_d(f) if f is not None else None for f in fl]
+        _grid_lib().oracle_grid_explicit_parts(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u),
+                                               _d(self.p), _d(tm), _d(ws), ptr[0], ptr[1], ptr[2])
+        return tm[0], ws, fl
 
     def u2p(self):
         c = np.zeros(3, np.int32)
