@@ -113,12 +113,123 @@ def cpu_baseline_implicit(P, pp, g20, G20, dt, args):
             "sample": "first %d cells of the same seeded batch, C oracle (gcc -O2, OpenMP)" % ns}
 
 
+# Algorithmic HBM bytes per thread-unit of each grid operator, counted as the
+# compulsory traffic of that operator with the metric as KORAL tabulates it
+# (20+20 doubles per cell or face side, 64 Christoffels per cell): every
+# primitive/conserved vector is one NV-vector of FP64, stencil neighbours
+# are shared through the caches.
+def grid_alg_bytes(kind, nv):
+    return {
+        "flux": (nv + 6 + 40 + nv) * 8,            # per face: prims, 6 speeds, face metric, flux
+        "u2p": (4 * nv + 40) * 8 + 4,              # per cell: u, p in/out, metric, 4 flags
+        "update": (nv + nv + 3 * nv + 40 + 64 + nv) * 8,  # per cell: p, u, 3 face fluxes, metric, Gamma, u
+        "wavespeeds": (nv + 40 + 18) * 8,          # per cell: prims, metric, 18 speeds
+        "emf": 9 * 8, "ct_faces": 9 * 8,           # per edge
+        "comb": 3 * 8,                             # per (cell, variable)
+        "bc_faces": (3 * nv + 40) * 8, "bc_edge": 4 * (3 * nv + 40) * 8, "bc_total": (3 * nv + 40) * 8,
+        "p2u": (2 * nv + 40) * 8,
+        "fixup": 8, "fixup_apply": 8,
+        "implicit": (4 * nv + 40 + 1) * 8 + 3,
+    }.get(kind, 0)
+
+
+def bench_grid(args, kind):
+    import numpy as np
+    import torch
+    from koral_amd import params as KP, problems as PR, grid as GR
+    from koral_amd.gridrun import DeviceGrid
+
+    if kind == "grmhd":
+        P = KP.params_torus_mhd()
+        n = tuple(args.grid) if args.grid else (128, 128, 64)
+        label = "3D GRMHD RK2IMEX step: MKS1 Kerr a=0.9 magnetised torus, MP5+HLL+flux-CT, u2p"
+    else:
+        P = KP.params_torus_rad()
+        n = tuple(args.grid) if args.grid else (192, 192, 96)
+        label = "3D GRRMHD+M1 RK2IMEX step: MKS1 Kerr a=0.9 torus with radiation, MP5+HLL+flux-CT, u2p, u2prad, implicit"
+    spec, p = PR.torus_setup(P, n=n, seed=args.seed)
+    stream = torch.cuda.Stream()
+    dg = DeviceGrid(P, spec, stream=stream.cuda_stream)
+    dg.upload(GR.to_aos(p))
+    dg.init()
+    for _ in range(args.warmup):
+        dg.step()
+    zones = n[0] * n[1] * n[2]
+    counters = np.zeros(6, np.int64)
+    dg.timing(enable=True, read=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dg.step()
+        counters += dg.counters
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = dg.timing(enable=False)
+    dom = max(tm, key=lambda k: tm[k][0])
+    ms, units, nl = tm[dom]
+    per_launch = ms / nl / 1e3
+    bytes_launch = grid_alg_bytes(dom, P.nv) * units / nl
+    achieved = bytes_launch / per_launch / 1e9
+    res = {
+        "metric": METRIC,
+        "value": zones * args.steps / wall,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded torus initial data)",
+        "config": {"workload": label, "grid": list(n), "nv": P.nv, "ng": spec.ng,
+                   "counters_per_step": (counters / args.steps).tolist(),
+                   "operator_ms_per_step": {k: v[0] / args.steps for k, v in sorted(tm.items(), key=lambda kv: -kv[1][0])}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom, "kernel_ms": per_launch * 1e3, "launches": nl,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_grid(P, kind, args)
+    return res
+
+
+def cpu_baseline_grid(P, kind, args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from koral_amd import problems as PR, grid as GR
+    n = tuple(args.cpu_grid)
+    spec, p = PR.torus_setup(P, n=n, seed=args.seed)
+    G = pyoracle.Grid(spec)
+    S = pyoracle.GridState(P, G, GR.to_aos(p))
+    S.init_u()
+    S.set_bc()
+    dt = 1. / S.tstepden()
+    t, steps, el = 0., 0, 0.
+    while steps < 1 or (el < args.cpu_seconds and steps < 50):
+        t0 = time.perf_counter()
+        tm, _ = S.step(t, dt)
+        el += time.perf_counter() - t0
+        t += dt
+        dt = 1. / tm
+        steps += 1
+    zones = n[0] * n[1] * n[2]
+    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d RK2IMEX steps of the same torus set-up at %dx%dx%d, serial C oracle (gcc -O2)"
+                      % (steps, n[0], n[1], n[2])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="implicit", choices=["implicit"])
+    ap.add_argument("--workload", default="implicit", choices=["implicit", "grmhd", "grrmhd"])
+    ap.add_argument("--grid", type=int, nargs=3, default=None)
+    ap.add_argument("--cpu-grid", type=int, nargs=3, default=[48, 48, 24])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cells", type=int, default=1 << 22)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
@@ -129,7 +240,7 @@ def main():
         print("bench.py runs on one GPU only: the single-GPU build holds the whole domain "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
-    res = bench_implicit(args)
+    res = bench_implicit(args) if args.workload == "implicit" else bench_grid(args, args.workload)
     print(json.dumps(res), flush=True)
 
 

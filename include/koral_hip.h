@@ -141,6 +141,16 @@ int koral_hip_grid_device_state(koral_hip_grid *g, double **p, double **u, signe
    No reference counterpart; used to localise parity differences. */
 int koral_hip_grid_download_work(koral_hip_grid *g, int which, double *out);
 
+/* Per-operator launch timing.  enable != 0 records an event pair around
+   every following launch on the grid's stream.  When any output is given the
+   call synchronises and returns, per operator kind k < nkinds, the summed
+   milliseconds, the threads launched (cells, faces or edges) and the number
+   of launches since the last call; the records are then cleared.  Kind names
+   come from koral_hip_grid_op_name (14 kinds).  No reference counterpart
+   (KORAL times whole steps only, ko.c:771). */
+int koral_hip_grid_timing(koral_hip_grid *g, int enable, double *ms, long *units, int *launches, int nkinds);
+const char *koral_hip_grid_op_name(int kind);
+
 /* u = p2u(p) in every cell (as set_initial_profile leaves the state) */
 int koral_hip_grid_p2u(koral_hip_grid *g);
 /* set_bc (finite.c:2752-3700): ghost cells from the boundary kinds, then the

@@ -15,6 +15,12 @@
 
 namespace kd {
 
+// Operator kinds (per-kind launch timing and names in the C-ABI)
+enum OpKind {
+  K_P2U, K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_FLUX, K_EMF, K_CT, K_UPDATE,
+  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_NKINDS
+};
+
 struct Box {
   int x0, y0, z0, nx, ny, nz;
   KD_HD long size() const { return (long)nx * ny * nz; }
@@ -60,6 +66,7 @@ KD_INL void p2u_at(const koral_params &P, const DGrid &G, const double *p, doubl
 // u = p2u(p) over a box (set_initial_profile leaves u consistent with p)
 template <int MAG, int RAD>
 struct OpP2U {
+  static constexpr int kind = K_P2U;
   koral_params P;
   DGrid G;
   const double *p;
@@ -80,6 +87,7 @@ struct OpP2U {
 // ghost cells outside the domain in exactly one dimension
 template <int MAG, int RAD>
 struct OpBcFaces {
+  static constexpr int kind = K_BC_FACES;
   koral_params P;
   DGrid G;
   double *p, *u;
@@ -133,6 +141,7 @@ struct OpBcFaces {
 // of the far edge ghosts, two diagonal averages) in the reference's order.
 template <int MAG, int RAD>
 struct OpBcEdge {
+  static constexpr int kind = K_BC_EDGE;
   koral_params P;
   DGrid G;
   double *p, *u;
@@ -202,6 +211,7 @@ struct OpBcEdge {
 // the eight NG^3 blocks at the domain corners: copies of the corner cell
 template <int MAG, int RAD>
 struct OpBcTotal {
+  static constexpr int kind = K_BC_TOTAL;
   koral_params P;
   DGrid G;
   double *p, *u;
@@ -231,6 +241,7 @@ struct OpBcTotal {
 // ---------------------------------------------------------------------------
 template <int MAG, int RAD>
 struct OpWavespeeds {
+  static constexpr int kind = K_WAVESPEEDS;
   koral_params P;
   DGrid G;
   const double *p;
@@ -330,6 +341,7 @@ KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__re
 
 template <int MAG, int RAD, int ORDER>
 struct OpFlux {
+  static constexpr int kind = K_FLUX;
   koral_params P;
   DGrid G;
   const double *p, *ws;
@@ -391,6 +403,7 @@ struct Flx {
 };
 
 struct OpEmf {
+  static constexpr int kind = K_EMF;
   DGrid G;
   Flx F;
   double *emf;
@@ -434,6 +447,7 @@ struct OpEmf {
 };
 
 struct OpCtFaces {
+  static constexpr int kind = K_CT;
   DGrid G;
   Flx F;
   const double *emf;
@@ -474,6 +488,7 @@ struct OpCtFaces {
 // ---------------------------------------------------------------------------
 template <int MAG, int RAD>
 struct OpUpdate {
+  static constexpr int kind = K_UPDATE;
   koral_params P;
   DGrid G;
   const double *p;
@@ -514,6 +529,7 @@ struct OpUpdate {
 // ---------------------------------------------------------------------------
 template <int MAG, int RAD>
 struct OpU2P {
+  static constexpr int kind = K_U2P;
   koral_params P;
   DGrid G;
   double *u, *p;
@@ -554,6 +570,7 @@ struct OpU2P {
 // ---------------------------------------------------------------------------
 template <int MAG, int RAD>
 struct OpFixup {
+  static constexpr int kind = K_FIXUP;
   koral_params P;
   DGrid G;
   const double *p;
@@ -617,6 +634,7 @@ struct OpFixup {
 
 template <int MAG>
 struct OpFixupApply {
+  static constexpr int kind = K_FIXUP_APPLY;
   DGrid G;
   double *p, *u;
   signed char *fl;
@@ -647,6 +665,7 @@ struct OpFixupApply {
 // ---------------------------------------------------------------------------
 template <int MAG>
 struct OpImplicit {
+  static constexpr int kind = K_IMPLICIT;
   koral_params P;
   DGrid G;
   double *u, *p;
@@ -681,6 +700,7 @@ struct OpImplicit {
 
 // RK2IMEX linear combination over the domain: out = a x + b y (+ c z)
 struct OpComb {
+  static constexpr int kind = K_COMB;
   DGrid G;
   int nv;
   double *out;

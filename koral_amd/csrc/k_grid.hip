@@ -4,6 +4,7 @@
 // grid C-ABI of include/koral_hip.h.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
+#include <stdio.h>
 #include <string.h>
 #include <vector>
 #include "koral_hip.h"
@@ -64,25 +65,67 @@ __global__ void __launch_bounds__(BS) k_count(Op op, int *out)
   }
 }
 
+// Optional per-launch timing: an event pair around every launch, summed per
+// operator kind when read back.
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Rec {
+    int kind;
+    long units;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t next()
+  {
+    if (used == pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  ~Timing()
+  {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 struct HipExec {
   hipStream_t s;
+  Timing *tm;
+  template <class Op, class L>
+  void timed(const Op &op, L launch)
+  {
+    long n = op.size();
+    if (n <= 0) return;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (tm && tm->on) {
+      a = tm->next();
+      b = tm->next();
+    }
+    if (a && b) (void)hipEventRecord(a, s);
+    launch(n);
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      tm->recs.push_back(Timing::Rec{Op::kind, n, a, b});
+    }
+  }
   template <class Op>
   void each(const Op &op)
   {
-    long n = op.size();
-    if (n > 0) k_each<Op><<<blocks_for(n), BS, 0, s>>>(op);
+    timed(op, [&](long n) { k_each<Op><<<blocks_for(n), BS, 0, s>>>(op); });
   }
   template <class Op>
   void max(const Op &op, double *out)
   {
-    long n = op.size();
-    if (n > 0) k_max<Op><<<blocks_for(n), BS, 0, s>>>(op, out);
+    timed(op, [&](long n) { k_max<Op><<<blocks_for(n), BS, 0, s>>>(op, out); });
   }
   template <class Op>
   void count(const Op &op, int *out)
   {
-    long n = op.size();
-    if (n > 0) k_count<Op><<<blocks_for(n), BS, 0, s>>>(op, out);
+    timed(op, [&](long n) { k_count<Op><<<blocks_for(n), BS, 0, s>>>(op, out); });
   }
 };
 
@@ -128,6 +171,7 @@ __global__ void k_flags_c2i(long ncell, const signed char *soa, int *aos)
 struct koral_hip_grid {
   GridBufs b{};
   hipStream_t s = nullptr;
+  Timing timing;
   std::vector<void *> allocs;
   double *stage = nullptr;  // AoS staging for upload/download
 
@@ -146,10 +190,27 @@ struct koral_hip_grid {
   {
     for (void *q : allocs) (void)hipFree(q);
   }
-  HipExec ex() const { return HipExec{s}; }
+  HipExec ex() { return HipExec{s, &timing}; }
 };
 
 namespace {
+
+// The explicit operator with both magnetic field and radiation faulted the
+// device once (illegal address) in an earlier build of these kernels and the
+// cause has not been identified; until it is, that combination is refused
+// rather than launched.  The same kernels without radiation (GRMHD) or
+// without field (radiation hydro) and the per-cell implicit solver are
+// enabled.
+int refuse_mag_rad(const koral_hip_grid *g, const char *what)
+{
+  if (g->b.P.mag && g->b.P.rad) {
+    char msg[256];
+    snprintf(msg, sizeof msg, "%s: field + radiation explicit operator is disabled on the device "
+                              "(unresolved device fault)", what);
+    return set_error(KORAL_E_STATE, msg);
+  }
+  return 0;
+}
 
 int zero_tsd(koral_hip_grid *g)
 {
@@ -321,6 +382,41 @@ int koral_hip_grid_download_work(koral_hip_grid *g, int which, double *out)
   return rc ? rc : check_hip(hipStreamSynchronize(g->s), "sync");
 }
 
+int koral_hip_grid_timing(koral_hip_grid *g, int enable, double *ms, long *units, int *launches, int nkinds)
+{
+  if (!g || nkinds < 0) return set_error(KORAL_E_ARG, "grid_timing: bad arguments");
+  Timing &T = g->timing;
+  int rc = 0;
+  if (ms || units || launches) {
+    rc = check_hip(hipStreamSynchronize(g->s), "timing sync");
+    for (int k = 0; k < nkinds; k++) {
+      if (ms) ms[k] = 0.;
+      if (units) units[k] = 0;
+      if (launches) launches[k] = 0;
+    }
+    for (const Timing::Rec &r : T.recs) {
+      if (rc || r.kind >= nkinds) continue;
+      float e = 0.f;
+      rc = check_hip(hipEventElapsedTime(&e, r.a, r.b), "event elapsed");
+      if (ms) ms[r.kind] += e;
+      if (units) units[r.kind] += r.units;
+      if (launches) launches[r.kind] += 1;
+    }
+  }
+  T.recs.clear();
+  T.used = 0;
+  T.on = enable != 0;
+  return rc;
+}
+
+const char *koral_hip_grid_op_name(int kind)
+{
+  static const char *names[K_NKINDS] = {"p2u", "bc_faces", "bc_edge", "bc_total", "wavespeeds", "flux",
+                                        "emf", "ct_faces", "update", "u2p", "fixup", "fixup_apply",
+                                        "implicit", "comb"};
+  return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "";
+}
+
 int koral_hip_grid_p2u(koral_hip_grid *g)
 {
   if (!g) return set_error(KORAL_E_ARG, "null grid");
@@ -352,7 +448,8 @@ int koral_hip_op_explicit(koral_hip_grid *g, double t, double dt, double *tstepd
 {
   (void)t;
   if (!g) return set_error(KORAL_E_ARG, "null grid");
-  int rc = zero_tsd(g);
+  int rc = refuse_mag_rad(g, "op_explicit");
+  if (!rc) rc = zero_tsd(g);
   if (rc) return rc;
   HipExec ex = g->ex();
   explicit_op(ex, g->b, dt);
@@ -380,7 +477,8 @@ int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt)
 int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstepdenmax, int *counters)
 {
   if (!g) return set_error(KORAL_E_ARG, "null grid");
-  int rc = zero_tsd(g);
+  int rc = refuse_mag_rad(g, "step_rk2imex");
+  if (!rc) rc = zero_tsd(g);
   if (!rc) rc = check_hip(hipMemsetAsync(g->b.counters, 0, 8 * sizeof(int), g->s), "memset counters");
   if (rc) return rc;
   HipExec ex = g->ex();

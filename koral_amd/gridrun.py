@@ -35,6 +35,7 @@ def _sig(L):
         "koral_hip_grid_download": [_vp, _vp, _vp, _vp],
         "koral_hip_grid_device_state": [_vp, _vp, _vp, _vp],
         "koral_hip_grid_download_work": [_vp, i, _vp],
+        "koral_hip_grid_timing": [_vp, i, _vp, _vp, _vp, i],
         "koral_hip_grid_p2u": [_vp],
         "koral_hip_set_bc": [_vp],
         "koral_hip_tstepden": [_vp, _vp],
@@ -46,8 +47,13 @@ def _sig(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i
+    L.koral_hip_grid_op_name.argtypes = [i]
+    L.koral_hip_grid_op_name.restype = ctypes.c_char_p
     L._grid_sigs = True
     return L
+
+
+NKINDS = 14
 
 
 class DeviceGrid:
@@ -126,6 +132,21 @@ class DeviceGrid:
             check(self.L.koral_hip_grid_download_work(self.h, d + 1, f.ctypes.data_as(_vp)), "download_work")
             fl.append(np.ascontiguousarray(np.moveaxis(f, 0, -1)))
         return np.ascontiguousarray(np.moveaxis(ws, 0, -1)), fl
+
+    def timing(self, enable=True, read=True):
+        """Per-operator launch timing.  Returns {name: (ms, threads, launches)}
+        for the launches since the previous call (read=True), then records
+        the following launches if enable."""
+        if not read:
+            check(self.L.koral_hip_grid_timing(self.h, int(enable), None, None, None, 0), "grid_timing")
+            return {}
+        ms = np.zeros(NKINDS)
+        units = np.zeros(NKINDS, np.int64)
+        nl = np.zeros(NKINDS, np.int32)
+        check(self.L.koral_hip_grid_timing(self.h, int(enable), ms.ctypes.data_as(_vp), units.ctypes.data_as(_vp),
+                                           nl.ctypes.data_as(_vp), NKINDS), "grid_timing")
+        return {self.L.koral_hip_grid_op_name(k).decode(): (float(ms[k]), int(units[k]), int(nl[k]))
+                for k in range(NKINDS) if nl[k]}
 
     def p2u(self):
         check(self.L.koral_hip_grid_p2u(self.h), "grid_p2u")

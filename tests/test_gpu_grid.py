@@ -169,3 +169,18 @@ def test_grid_step_parity(oracle, kind, nsteps):
         t += dt
         dt = 1. / tm_o
         dg.dt = dt
+
+
+def test_field_radiation_explicit_refused():
+    """The field + radiation explicit operator is refused (no launch) while
+    its device fault is unresolved."""
+    from koral_amd.gridrun import DeviceGrid
+    from koral_amd._lib import KoralError
+    P, spec, p = _setup("torus_rad")
+    dg = DeviceGrid(P, spec)
+    dg.upload(p)
+    with pytest.raises(KoralError, match="disabled"):
+        dg.explicit(0., 1e-3)
+    with pytest.raises(KoralError, match="disabled"):
+        dg.dt = 1e-3
+        dg.step()
