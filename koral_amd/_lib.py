@@ -24,6 +24,10 @@ def lib():
     if not os.path.exists(LIBPATH):
         raise ImportError("libkoral_hip.so is not built (run __graft_entry__.build() or "
                           "python koral_amd/build.py); the HIP path has no CPU fallback")
+    # torch ships its own HIP runtime under the same soname; loading it first
+    # makes this library bind to that one, so a process never holds two HIP
+    # runtimes (the second would leave torch without a device).
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIBPATH)
     L.koral_hip_last_error.restype = ctypes.c_char_p
     L.koral_hip_version.restype = ctypes.c_int
