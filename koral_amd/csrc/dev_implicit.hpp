@@ -182,36 +182,34 @@ KD_INL int complete_state(const koral_params &P, double *pp, double *uu, const d
   return r;
 }
 
-// solve_implicit_lab_4dprim, rad.c:376-1218
+// One iteration of the damped Newton loop of solve_implicit_lab_4dprim
+// (rad.c:376-1218) on the iterate pp (the attempt starts from pp = pp00,
+// iter = 0).  Returns IMP_CONT to iterate again, IMP_OK with the final state
+// in pp, or IMP_FAIL; iter counts the iterations of the attempt as the
+// reference does.
+enum { IMP_FAIL = -1, IMP_CONT = 0, IMP_OK = 1 };
+
 template <int MAG, int RAD>
-KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
-                            int whichprim, int whicheq, int whichframe, double *pp, double global_time, int &iters)
+KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
+                           int whichprim, int whicheq, int whichframe, double *pp, double global_time, int &iter)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   constexpr int EE = Lay<MAG, RAD>::EE;
   double pp0[NV], ppp[NV], uu[NV];
   double f1[4], f2[4], xxx[4], J[16], iJ[16], err, errbase;
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    pp0[i] = pp00[i];
-    pp[i] = pp0[i];
-  }
+  for (int i = 0; i < NV; i++) pp0[i] = pp00[i];
   const double EPS = P.radimpeps;
   const double CONV = (whicheq == ENTROPYEQ) ? P.radimpentrconv : P.radimpconv;
   const int MAXITER = P.radimpmaxiter;
   const int sh = (whichprim == P_MHD) ? 1 : EE;
-  int iter = 0, failed = 0;
-  iters = 0;
-  for (;;) {
-    iter++;
+  int failed = 0;
+  iter++;
 #pragma unroll
-    for (int i = 0; i < NV; i++) ppp[i] = pp[i];
-    if (f_implicit<MAG, RAD>(P, pp, uu00, pp0, dt, g, f1, whichprim, whicheq, whichframe, err) < 0) {
-      iters = iter;
-      return -1;
-    }
-    errbase = err;
-    if (err < CONV) break;
+  for (int i = 0; i < NV; i++) ppp[i] = pp[i];
+  if (f_implicit<MAG, RAD>(P, pp, uu00, pp0, dt, g, f1, whichprim, whicheq, whichframe, err) < 0) return IMP_FAIL;
+  errbase = err;
+  if (err >= CONV) {
     for (int j = 0; j < 4; j++) {
       double sign = -1.;
       const double xj = ppp[j + sh];
@@ -239,13 +237,9 @@ KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double 
         for (int i = 0; i < 4; i++) J[i * 4 + j] = (f2[i] - f1[i]) / (xp - xj);
         break;
       }
-      if (failed) break;
+      if (failed) return IMP_FAIL;
     }
-    if (failed) break;
-    if (inverse44(J, iJ) < 0) {
-      failed = 1;
-      break;
-    }
+    if (inverse44(J, iJ) < 0) return IMP_FAIL;
     double xiapp = 1.;
     for (;;) {
 #pragma unroll
@@ -266,11 +260,14 @@ KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double 
       } else {
         xiapp /= P.dampingfactor;
       }
-      if (xiapp < P.maxradimpdamping) {
+      // rad.c:973; a NaN factor (0/0 when the iterate and the step are both
+      // zero) ends the attempt where the reference would never leave
+      if (!(xiapp >= P.maxradimpdamping)) {
         failed = 1;
         break;
       }
     }
+    bool conv_rel = false;
     if (P.convrel_enabled && failed == 0 && global_time < 100.) {
       double f3[4];
       f3[0] = fabs((pp[sh] - ppp[sh]) / ppp[sh]);
@@ -284,15 +281,49 @@ KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double 
         CR = P.radimpconvrelentr;
         CRE = P.radimpconvrelentrerr;
       }
-      if (f3[0] < CR && f3[1] < CR && f3[2] < CR && f3[3] < CR && errbase < CRE) break;
+      conv_rel = f3[0] < CR && f3[1] < CR && f3[2] < CR && f3[3] < CR && errbase < CRE;
     }
-    if (iter > MAXITER || failed == 1) break;
+    if (!conv_rel) return (iter > MAXITER || failed == 1) ? IMP_FAIL : IMP_CONT;
   }
-  iters = iter;
-  if (iter > MAXITER || failed == 1) return -1;
+  // converged: the reference still rejects a solution past MAXITER
+  if (iter > MAXITER) return IMP_FAIL;
   int u2pret = complete_state<MAG, RAD>(P, pp, uu, uu00, g, whichprim, true);
-  if (u2pret < -1) return -1;
-  return 0;
+  return (u2pret < -1) ? IMP_FAIL : IMP_OK;
+}
+
+// solve_implicit_lab_4dprim, rad.c:376-1218: one attempt from pp00
+template <int MAG, int RAD>
+KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
+                        int whichprim, int whicheq, int whichframe, double *pp, double global_time, int &iters)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+#pragma unroll
+  for (int i = 0; i < NV; i++) pp[i] = pp00[i];
+  int iter = 0, st;
+  do {
+    st = imp_newton_step<MAG, RAD>(P, uu00, pp00, g, dt, whichprim, whicheq, whichframe, pp, global_time, iter);
+  } while (st == IMP_CONT);
+  iters = iter;
+  return st == IMP_OK ? 0 : -1;
+}
+
+// attempt a of the fallback chain (rad.c:1389-1464): energy LAB, energy FF,
+// swapped LAB, swapped FF, entropy LAB, entropy swapped
+KD_INL void imp_attempt(int a, int startwith, int &wp, int &we, int &wf)
+{
+  const int other = (startwith == P_RAD) ? P_MHD : P_RAD;
+  wp = (a == 2 || a == 3 || a == 5) ? other : startwith;
+  we = (a >= 4) ? ENTROPYEQ : ENERGYEQ;
+  wf = (a == 1 || a == 3) ? FF : LAB;
+}
+
+// the fallback chain starts from RAD primitives when the fluid-frame
+// radiation energy is small against the gas energy (rad.c:1340, 1381)
+template <int MAG, int RAD>
+KD_INL int imp_startwith(const koral_params &P, const double *pp00, const Geom &g)
+{
+  double Ehat = -ff_Rtt<MAG, RAD>(pp00, g, nullptr);
+  return (Ehat < P.radimplicitthreshold * pp00[1]) ? P_RAD : P_MHD;
 }
 
 struct ImpInfo {
@@ -303,23 +334,17 @@ struct ImpInfo {
 // rad.c:2758-2795).  Returns 0 on success with uu/pp updated.
 template <int MAG, int RAD>
 KD_FUN int solve_implicit_lab(const koral_params &P, double *uu, double *pp, const Geom &g, double dt,
-                                  double global_time, ImpInfo &info)
+                              double global_time, ImpInfo &info)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   double pp00[NV], ppn[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) pp00[i] = pp[i];
-  double Ehat = -ff_Rtt<MAG, RAD>(pp00, g, nullptr);
-  const int startwith = (Ehat < P.radimplicitthreshold * pp00[1]) ? P_RAD : P_MHD;
-  const int other = (startwith == P_RAD) ? P_MHD : P_RAD;
+  const int startwith = imp_startwith<MAG, RAD>(P, pp00, g);
   int ret = -1, total = 0, it = 0, a = 0;
   int wp = startwith, we = ENERGYEQ, wf = LAB;
   for (a = 0; a < 6; a++) {
-    // attempt order: energy LAB, energy FF, swapped LAB, swapped FF,
-    // entropy LAB, entropy swapped (rad.c:1389-1464)
-    wp = (a == 2 || a == 3 || a == 5) ? other : startwith;
-    we = (a >= 4) ? ENTROPYEQ : ENERGYEQ;
-    wf = (a == 1 || a == 3) ? FF : LAB;
+    imp_attempt(a, startwith, wp, we, wf);
     ret = solve_4dprim<MAG, RAD>(P, uu, pp00, g, dt, wp, we, wf, ppn, global_time, it);
     total += it;
     if (ret == 0) break;

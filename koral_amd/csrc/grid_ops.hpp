@@ -8,6 +8,9 @@
 //   ex.each(op)          run op(i) for every i
 //   ex.max(op, out)      *out = max(*out, max_i op(i))     (op(i) >= 0)
 //   ex.count(op, out3)   out3[k] += sum_i op(i).v[k], k < 3
+//   ex.implicit_cells(op, out3)   the implicit sweep OpImplicit with its
+//                        counts (the HIP executor runs it as one persistent
+//                        launch whose lanes take cells from a work counter)
 // The library's executor launches one HIP kernel per call on the grid's
 // stream (k_grid.hip).
 #pragma once
@@ -741,6 +744,7 @@ struct GridBufs {
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
   double *tsd;    // time-step density max (one double)
   int *counters;  // step counters [6]
+  unsigned int *queue;  // work counter of the persistent implicit sweep
 };
 
 // Ghost-inclusive layout of a grid description (a collapsed dimension,
@@ -899,9 +903,9 @@ void implicit_op(Ex &ex, const GridBufs &s, double t, double dt)
 {
   if (!s.P.rad) return;
   if (s.P.mag)
-    ex.count(OpImplicit<1>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+    ex.implicit_cells(OpImplicit<1>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
   else
-    ex.count(OpImplicit<0>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+    ex.implicit_cells(OpImplicit<0>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
   fixup(ex, s, F_RADIMPFIXUP);
 }
 

@@ -5,7 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 #include "koral_hip.h"
-#include "dev_implicit.hpp"
+#include <mutex>
+#include "imp_persist.hpp"
 #include "capi_util.hpp"
 
 using namespace kd;
@@ -132,43 +133,94 @@ __global__ void __launch_bounds__(256) k_Gi(koral_params P, long n, const double
   for (int k = 0; k < 4; k++) Gi[k * n + i] = G4[k];
 }
 
+// cell source of the batched implicit solve: SoA arrays with stride n
 template <int MAG, int RAD>
-__global__ void __launch_bounds__(256) k_implicit(koral_params P, long n, double *__restrict__ uu,
-                                                  double *__restrict__ pp, const double *__restrict__ g20,
-                                                  const double *__restrict__ G20, const double *__restrict__ dt,
-                                                  double global_time, int *__restrict__ flags,
-                                                  int *__restrict__ info)
-{
-  constexpr int NV = Lay<MAG, RAD>::NV;
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Geom g;
-  load_point<MAG, RAD>(g20, G20, n, i, g);
-  double p[NV], u[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    p[v] = pp[v * n + i];
-    u[v] = uu[v * n + i];
-  }
-  ImpInfo inf;
-  int r = solve_implicit_lab<MAG, RAD>(P, u, p, g, dt[i], global_time, inf);
-  if (r == 0) {
+struct BatchCells {
+  long n;
+  double *uu, *pp;
+  const double *g20, *G20, *dt;
+  int *flags, *info;
+  __device__ __forceinline__ void load(long c, double *uu00, double *pp00, Geom &g, double &dtc) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    load_point<MAG, RAD>(g20, G20, n, c, g);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      pp[v * n + i] = p[v];
-      uu[v * n + i] = u[v];
+      pp00[v] = pp[v * n + c];
+      uu00[v] = uu[v * n + c];
     }
+    dtc = dt[c];
   }
-  flags[F_RADSOURCETYPE * n + i] = 3;
-  flags[F_RADFIXUP * n + i] = (r == 0) ? 0 : -1;
-  flags[F_RADIMPFIXUP * n + i] = (r == 0) ? 0 : -1;
-  info[0 * n + i] = inf.iters;
-  info[1 * n + i] = inf.attempt;
-  info[2 * n + i] = inf.whichprim;
-  info[3 * n + i] = inf.whicheq;
-  info[4 * n + i] = inf.whichframe;
-  info[5 * n + i] = inf.total;
+  __device__ __forceinline__ void done_ok(long c, const double *u, const double *p, int iters, int attempt, int wp,
+                                          int we, int wf, int total) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      pp[v * n + c] = p[v];
+      uu[v * n + c] = u[v];
+    }
+    flags[F_RADSOURCETYPE * n + c] = 3;
+    flags[F_RADFIXUP * n + c] = 0;
+    flags[F_RADIMPFIXUP * n + c] = 0;
+    info[0 * n + c] = iters;
+    info[1 * n + c] = attempt;
+    info[2 * n + c] = wp;
+    info[3 * n + c] = we;
+    info[4 * n + c] = wf;
+    info[5 * n + c] = total;
+  }
+  __device__ __forceinline__ void done_fail(long c, int total) const
+  {
+    flags[F_RADSOURCETYPE * n + c] = 3;
+    flags[F_RADFIXUP * n + c] = -1;
+    flags[F_RADIMPFIXUP * n + c] = -1;
+    info[0 * n + c] = 0;
+    info[1 * n + c] = -1;
+    info[2 * n + c] = 0;
+    info[3 * n + c] = 0;
+    info[4 * n + c] = 0;
+    info[5 * n + c] = total;
+  }
+};
+
+constexpr int IMP_BS = 256;
+
+template <int MAG, int RAD>
+__global__ void __launch_bounds__(IMP_BS) k_implicit(koral_params P, BatchCells<MAG, RAD> src,
+                                                     unsigned int *queue, double global_time)
+{
+  implicit_persistent<MAG, RAD>(P, src, src.n, queue, global_time, nullptr);
 }
+
+// Work counters of the persistent implicit launch: one allocation per device
+// (made on the first call), a ring of slots so calls on different streams
+// never share a counter; each call clears its slot with a kernel on its
+// stream before the launch.
+constexpr int QRING = 1024;
+struct QueueRing {
+  unsigned int *slots = nullptr;
+  unsigned int next = 0;
+};
+std::mutex qmu;
+QueueRing qrings[64];
+
+int queue_slot(unsigned int **out)
+{
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= 64) return set_error(KORAL_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(qmu);
+  QueueRing &r = qrings[dev];
+  if (!r.slots) {
+    rc = check_hip(hipMalloc(&r.slots, QRING * sizeof(unsigned int)), "hipMalloc queue");
+    if (rc) return rc;
+  }
+  *out = r.slots + (r.next++ % QRING);
+  return 0;
+}
+
 
 inline dim3 grid_for(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
@@ -249,11 +301,24 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   if (check_params(P) || n < 0) return KORAL_E_ARG;
   if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
   if (n == 0) return 0;
+  if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "implicit batch larger than 2^31 cells");
   hipStream_t s = (hipStream_t)stream;
-  if (P->mag)
-    k_implicit<1, 1><<<grid_for(n, 256), 256, 0, s>>>(*P, n, uu, pp, g20, G20, dt, global_time, flags, info);
-  else
-    k_implicit<0, 1><<<grid_for(n, 256), 256, 0, s>>>(*P, n, uu, pp, g20, G20, dt, global_time, flags, info);
+  unsigned int *q = nullptr;
+  int rc = queue_slot(&q);
+  if (rc) return rc;
+  k_zero_u32<<<1, 1, 0, s>>>(q);
+  unsigned blocks = 0;
+  if (P->mag) {
+    rc = resident_blocks(k_implicit<1, 1>, IMP_BS, n, &blocks);
+    if (rc) return rc;
+    BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
+    k_implicit<1, 1><<<blocks, IMP_BS, 0, s>>>(*P, src, q, global_time);
+  } else {
+    rc = resident_blocks(k_implicit<0, 1>, IMP_BS, n, &blocks);
+    if (rc) return rc;
+    BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
+    k_implicit<0, 1><<<blocks, IMP_BS, 0, s>>>(*P, src, q, global_time);
+  }
   return check_launch("koral_hip_implicit_batch");
 }
 

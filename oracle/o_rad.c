@@ -488,7 +488,9 @@ int o_solve_implicit_4dprim(const oracle_params *P, const double *uu00, const do
       } else {
         xiapp /= P->dampingfactor;
       }
-      if (xiapp < P->maxradimpdamping) {
+      /* rad.c:973; a NaN factor (0/0 when the iterate and the step are both
+         zero) ends the attempt where the reference would never leave */
+      if (!(xiapp >= P->maxradimpdamping)) {
         failed = 1;
         break;
       }
