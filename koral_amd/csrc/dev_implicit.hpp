@@ -9,10 +9,11 @@ namespace kd {
 
 // f_implicit_lab_4dprim: returns -1 on failure, 1 if the M1 inversion was
 // capped, 0 otherwise.  pp0 may be modified (fluid-frame entropy equation).
-template <int MAG, int RAD>
+template <int MAG, int RAD, int WP, int WE, int WF>
 KD_FUN int f_implicit(const koral_params &P, const double *ppin, const double *uu0, double *pp0, double dt,
-                          const Geom &g, double *f, int whichprim, int whicheq, int whichframe, double &err0)
+                      const Geom &g, double *f, double &err0)
 {
+  constexpr int whichprim = WP, whicheq = WE, whichframe = WF;
   constexpr int NV = Lay<MAG, RAD>::NV;
   constexpr int EE = Lay<MAG, RAD>::EE;
   double gdetu = g.gdet;
@@ -50,7 +51,6 @@ KD_FUN int f_implicit(const koral_params &P, const double *ppin, const double *u
   calc_Gi<MAG, RAD>(P, pp, g, Gi, 1);
   lower(Gi, Gi, g);
   const int base = (whichprim == P_MHD) ? 1 : EE;
-  const double sgn_src = (whichprim == P_MHD) ? -1. : 1.;
 #pragma unroll
   for (int k = 1; k < 4; k++) {
     if (whichprim == P_MHD)
@@ -61,7 +61,6 @@ KD_FUN int f_implicit(const koral_params &P, const double *ppin, const double *u
                  ? fabs(f[k]) / (fabs(uu[base + k]) + fabs(uu0[base + k]) + fabs(dt * gdetu * Gi[k]))
                  : 0.;
   }
-  (void)sgn_src;
   if (whichframe == LAB) {
     int iv = (whicheq == ENERGYEQ) ? base : 5;
     if (whichprim == P_RAD)
@@ -148,10 +147,11 @@ KD_INL int inverse44(const double *a, double *ia)
 
 // Newton-update helper: the "other half" of the state follows from
 // conservation of U_gas+U_rad and an inversion (rad.c:880-950).
-template <int MAG, int RAD>
-KD_INL int complete_state(const koral_params &P, double *pp, double *uu, const double *uu0,
-                                              const Geom &g, int whichprim, bool final_pass)
+template <int MAG, int RAD, int WP>
+KD_INL int complete_state(const koral_params &P, double *pp, double *uu, const double *uu0, const Geom &g,
+                          bool final_pass)
 {
+  constexpr int whichprim = WP;
   constexpr int EE = Lay<MAG, RAD>::EE;
   double gdetu = g.gdet;
   if (whichprim == P_MHD) {
@@ -189,10 +189,11 @@ KD_INL int complete_state(const koral_params &P, double *pp, double *uu, const d
 // reference does.
 enum { IMP_FAIL = -1, IMP_CONT = 0, IMP_OK = 1 };
 
-template <int MAG, int RAD>
+template <int MAG, int RAD, int WP, int WE, int WF>
 KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const double *pp00, const Geom &g, double dt,
-                           int whichprim, int whicheq, int whichframe, double *pp, double global_time, int &iter)
+                           double *pp, double global_time, int &iter)
 {
+  constexpr int whichprim = WP, whicheq = WE;
   constexpr int NV = Lay<MAG, RAD>::NV;
   constexpr int EE = Lay<MAG, RAD>::EE;
   double pp0[NV], ppp[NV], uu[NV];
@@ -207,24 +208,31 @@ KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const doub
   iter++;
 #pragma unroll
   for (int i = 0; i < NV; i++) ppp[i] = pp[i];
-  if (f_implicit<MAG, RAD>(P, pp, uu00, pp0, dt, g, f1, whichprim, whicheq, whichframe, err) < 0) return IMP_FAIL;
+  if (f_implicit<MAG, RAD, WP, WE, WF>(P, pp, uu00, pp0, dt, g, f1, err) < 0) return IMP_FAIL;
   errbase = err;
   if (err >= CONV) {
+    // Jacobian by one-sided differences, column j from the iterate with
+    // variable j + sh moved to xp (the loop over j is not unrolled, so the
+    // arrays are addressed through selects and stay in registers)
     for (int j = 0; j < 4; j++) {
       double sign = -1.;
-      const double xj = ppp[j + sh];
+      double xj = ppp[sh];
+#pragma unroll
+      for (int k = 1; k < 4; k++) xj = (j == k) ? ppp[k + sh] : xj;
+      const double gjj = (j == 1) ? g.g[1][1] : ((j == 2) ? g.g[2][2] : g.g[3][3]);
       for (;;) {
         double del;
         if (j == 0) {
           del = sign * EPS * xj;
         } else {
-          double veleps = EPS * sgn(xj) * mx(1.e-6 / sqrt(g.g[j][j]), fabs(xj));
+          double veleps = EPS * sgn(xj) * mx(1.e-6 / sqrt(gjj), fabs(xj));
           del = (xj >= 0.) ? sign * veleps : -sign * veleps;
         }
         const double xp = xj + del;
-        pp[j + sh] = xp;
-        int fret = f_implicit<MAG, RAD>(P, pp, uu00, pp0, dt, g, f2, whichprim, whicheq, whichframe, err);
-        pp[j + sh] = xj;
+        double ppt[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) ppt[k] = (k == j + sh) ? xp : pp[k];
+        int fret = f_implicit<MAG, RAD, WP, WE, WF>(P, ppt, uu00, pp0, dt, g, f2, err);
         if (fret < 0) {
           if (sign > 0.) {
             failed = 1;
@@ -234,7 +242,12 @@ KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const doub
           continue;
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) J[i * 4 + j] = (f2[i] - f1[i]) / (xp - xj);
+        for (int i = 0; i < 4; i++) {
+          const double d = (f2[i] - f1[i]) / (xp - xj);
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if (k == j) J[i * 4 + k] = d;
+        }
         break;
       }
       if (failed) return IMP_FAIL;
@@ -250,7 +263,7 @@ KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const doub
         for (int j = 0; j < 4; j++) xxx[i] -= xiapp * iJ[i * 4 + j] * f1[j];
 #pragma unroll
       for (int i = 0; i < 4; i++) pp[i + sh] = xxx[i];
-      int u2pret = complete_state<MAG, RAD>(P, pp, uu, uu00, g, whichprim, false);
+      int u2pret = complete_state<MAG, RAD, WP>(P, pp, uu, uu00, g, false);
       double edenmin = ppp[sh] / P.maxenchangedown;
       double edenmax = ppp[sh] * P.maxenchangeup;
       if (xxx[0] > edenmin && xxx[0] < edenmax && u2pret >= -1) break;
@@ -287,8 +300,26 @@ KD_FUN int imp_newton_step(const koral_params &P, const double *uu00, const doub
   }
   // converged: the reference still rejects a solution past MAXITER
   if (iter > MAXITER) return IMP_FAIL;
-  int u2pret = complete_state<MAG, RAD>(P, pp, uu, uu00, g, whichprim, true);
+  int u2pret = complete_state<MAG, RAD, WP>(P, pp, uu, uu00, g, true);
   return (u2pret < -1) ? IMP_FAIL : IMP_OK;
+}
+
+// imp_newton_step for attempt parameters known at run time: the six
+// (primitives, equation, frame) kinds the fallback chain uses
+template <int MAG, int RAD>
+KD_INL int imp_newton_step_rt(const koral_params &P, const double *uu00, const double *pp00, const Geom &g,
+                              double dt, int wp, int we, int wf, double *pp, double global_time, int &iter)
+{
+#define KD_STEP(A, B, C) imp_newton_step<MAG, RAD, A, B, C>(P, uu00, pp00, g, dt, pp, global_time, iter)
+  if (wp == P_MHD) {
+    if (we == ENERGYEQ) return wf == LAB ? KD_STEP(P_MHD, ENERGYEQ, LAB) : KD_STEP(P_MHD, ENERGYEQ, FF);
+    if (wf == LAB) return KD_STEP(P_MHD, ENTROPYEQ, LAB);
+  } else {
+    if (we == ENERGYEQ) return wf == LAB ? KD_STEP(P_RAD, ENERGYEQ, LAB) : KD_STEP(P_RAD, ENERGYEQ, FF);
+    if (wf == LAB) return KD_STEP(P_RAD, ENTROPYEQ, LAB);
+  }
+#undef KD_STEP
+  return IMP_FAIL;  // the chain never asks for the fluid-frame entropy equation
 }
 
 // solve_implicit_lab_4dprim, rad.c:376-1218: one attempt from pp00
@@ -301,7 +332,7 @@ KD_FUN int solve_4dprim(const koral_params &P, const double *uu00, const double 
   for (int i = 0; i < NV; i++) pp[i] = pp00[i];
   int iter = 0, st;
   do {
-    st = imp_newton_step<MAG, RAD>(P, uu00, pp00, g, dt, whichprim, whicheq, whichframe, pp, global_time, iter);
+    st = imp_newton_step_rt<MAG, RAD>(P, uu00, pp00, g, dt, whichprim, whicheq, whichframe, pp, global_time, iter);
   } while (st == IMP_CONT);
   iters = iter;
   return st == IMP_OK ? 0 : -1;

@@ -9,8 +9,8 @@
 //   ex.max(op, out)      *out = max(*out, max_i op(i))     (op(i) >= 0)
 //   ex.count(op, out3)   out3[k] += sum_i op(i).v[k], k < 3
 //   ex.implicit_cells(op, out3)   the implicit sweep OpImplicit with its
-//                        counts (the HIP executor runs it as one persistent
-//                        launch whose lanes take cells from a work counter)
+//                        counts (the HIP executor runs it as six persistent
+//                        per-attempt passes over compacted work lists)
 // The library's executor launches one HIP kernel per call on the grid's
 // stream (k_grid.hip).
 #pragma once
@@ -744,7 +744,8 @@ struct GridBufs {
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
   double *tsd;    // time-step density max (one double)
   int *counters;  // step counters [6]
-  unsigned int *queue;  // work counter of the persistent implicit sweep
+  unsigned *imp_list[2][2];  // work lists of the implicit passes (device executor)
+  unsigned *imp_ctr;         // their counters
 };
 
 // Ghost-inclusive layout of a grid description (a collapsed dimension,

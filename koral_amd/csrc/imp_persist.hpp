@@ -1,58 +1,126 @@
-// Persistent implicit-source solve: every lane of a resident wave owns one
-// cell at a time and advances it one damped-Newton iteration per pass of the
-// loop (imp_newton_step), walking the six-attempt fallback chain of
-// solve_implicit_lab (rad.c:1225-1570).  A lane whose cell has finished
-// (solved or out of attempts) takes the next cell from a global work
-// counter: the wave ballots its idle lanes, one lane claims that many cells
-// with a single atomic add, and each idle lane takes its rank among them
-// (prefix count of the ballot).  Cells need 1 to 306 iterations, so a lane
-// never waits for the slowest cell of its wave; the wave leaves when the
-// counter is exhausted and none of its lanes holds a cell.
+// The implicit-source solve of many cells as six passes, one per attempt of
+// solve_implicit_lab's fallback chain (rad.c:1389-1464: energy LAB, energy
+// FF, swapped LAB, swapped FF, entropy LAB, entropy swapped).
 //
-// Per-iteration state kept in registers: the iterate (NV doubles), the cell
-// index and the attempt/iteration counters.  The cell's inputs (U, P, the
-// metric, dt) are re-read from memory every iteration.
+// Pass a works on the cells still unsolved after a - 1 attempts, held in two
+// lists by the primitives attempt a iterates on (MHD or RAD).  Each list is
+// solved by its own kernel, specialised at compile time for the attempt's
+// (primitives, equation, frame), so the lanes of a wave run the same branch
+// of the residual.  The kernels are persistent: a lane owns one cell at a
+// time and advances it one damped-Newton iteration per pass of its loop
+// (imp_newton_step); a lane whose cell is done takes the next list entry
+// from a work counter (the wave ballots its idle lanes, one lane claims that
+// many entries with one atomic add, each idle lane takes its rank).  A cell
+// that fails the attempt is appended, by the same ballot/prefix-count
+// scheme, to the list of the next attempt.  Cells need 1 to 51 iterations
+// per attempt, so no lane waits for the slowest cell of its wave.
 //
-// Src provides, for a work index c in [0, n):
-//   load(c, uu00, pp00, g, dt)
-//   done_ok(c, uu, pp, iters, attempt, whichprim, whicheq, whichframe, total)
-//   done_fail(c, total)
+// Src provides, for a cell index c:
+//   load(c, uu00, pp00, g, dt)     inputs of the solve
+//   init(c)                        before the first attempt
+//   add_iters(c, it)               after every attempt
+//   done_ok(c, uu, pp, it, attempt, whichprim, whicheq, whichframe)
+//   done_fail(c)                   after the sixth failed attempt
 #pragma once
 #include "dev_implicit.hpp"
 #include "capi_util.hpp"
 
 namespace kd {
 
+// Device buffers of the passes: two list pairs (ping-pong, each n entries per
+// primitive kind) and IMP_NCTR counters.  Counter 4 a + k: k = 0, 1 the list
+// lengths for attempt a (MHD, RAD); k = 2, 3 the work counters of its two
+// kernels.
+constexpr int IMP_NCTR = 32;
+struct ImpBufs {
+  unsigned *list[2][2];
+  unsigned *ctr;
+};
+
+KD_INL int imp_other(int wp) { return wp == P_RAD ? P_MHD : P_RAD; }
+
+// primitives of attempt a + 1 for a cell iterating on wp in attempt a
+KD_INL int imp_next_wp(int a, int wp)
+{
+  const bool swapped = (a == 2 || a == 3 || a == 5);
+  const int startwith = swapped ? imp_other(wp) : wp;
+  const bool swapped_next = (a + 1 == 2 || a + 1 == 3 || a + 1 == 5);
+  return swapped_next ? imp_other(startwith) : startwith;
+}
+
+// append v to list when pred holds; every lane of the wave calls it
+__device__ __forceinline__ void wave_append(bool pred, unsigned v, unsigned *list, unsigned *len)
+{
+  const unsigned long long m = __ballot(pred);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(len, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// sorts the n cells into the two attempt-0 lists by the primitives the
+// chain starts with
 template <int MAG, int RAD, class Src>
-__device__ __forceinline__ void implicit_persistent(const koral_params &P, const Src &src, long n,
-                                                    unsigned int *queue, double global_time, int *counts)
+__global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, long n, ImpBufs b)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int wp = 0;
+  if (c < n) {
+    double uu00[NV], pp00[NV], dt;
+    Geom g;
+    src.load(c, uu00, pp00, g, dt);
+    wp = imp_startwith<MAG, RAD>(P, pp00, g);
+    src.init(c);
+  }
+  wave_append(wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
+  wave_append(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
+}
+
+// attempt a on the cells of one list, all iterating on primitives WP with
+// equation WE in frame WF
+template <int MAG, int RAD, int WP, int WE, int WF, class Src>
+__global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, double global_time,
+                                                  int *counts)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const int lane = threadIdx.x & 63;
-  bool active = false, fresh = false, exhausted = false;
-  long c = 0;
-  int a = 0, iter = 0, total = 0, startwith = P_MHD;
+  const int kind = (WP == P_MHD) ? 0 : 1;
+  const unsigned *list = b.list[a & 1][kind];
+  unsigned *out0 = b.list[(a + 1) & 1][0], *out1 = b.list[(a + 1) & 1][1];
+  const long n = b.ctr[4 * a + kind];
+  unsigned *queue = &b.ctr[4 * a + 2 + kind];
+  unsigned *out_len = &b.ctr[4 * (a + 1)];
+  const int next_kind = (imp_next_wp(a, WP) == P_MHD) ? 0 : 1;
+  bool active = false, fresh = false, exhausted = n == 0;
+  unsigned c = 0, c_push = 0;
+  bool push = false;
+  int iter = 0;
   int n_ok = 0, n_it = 0, n_bad = 0;
   double pp[NV];
   for (;;) {
+    // failures of the last iteration go to the next attempt's list
+    wave_append(push && next_kind == 0, c_push, out0, &out_len[0]);
+    wave_append(push && next_kind == 1, c_push, out1, &out_len[1]);
+    push = false;
     if (!exhausted) {
       const unsigned long long need = __ballot(!active);
       if (need) {
         const int cnt = __popcll(need);
         const int leader = __ffsll((unsigned long long)need) - 1;
-        unsigned int base = 0;
-        if (lane == leader) base = atomicAdd(queue, (unsigned int)cnt);
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(queue, (unsigned)cnt);
         base = __shfl(base, leader);
         if ((long)base + cnt >= n) exhausted = true;
         if (!active) {
-          const long cc = (long)base + __popcll(need & ((1ull << lane) - 1ull));
-          if (cc < n) {
-            c = cc;
+          const long k = (long)base + __popcll(need & ((1ull << lane) - 1ull));
+          if (k < n) {
+            c = list[k];
             active = true;
             fresh = true;
-            a = 0;
-            iter = 0;
-            total = 0;
           }
         }
       }
@@ -63,32 +131,28 @@ __device__ __forceinline__ void implicit_persistent(const koral_params &P, const
       Geom g;
       src.load(c, uu00, pp00, g, dt);
       if (fresh) {
-        startwith = imp_startwith<MAG, RAD>(P, pp00, g);
 #pragma unroll
         for (int i = 0; i < NV; i++) pp[i] = pp00[i];
+        iter = 0;
         fresh = false;
       }
-      int wp, we, wf;
-      imp_attempt(a, startwith, wp, we, wf);
-      const int st = imp_newton_step<MAG, RAD>(P, uu00, pp00, g, dt, wp, we, wf, pp, global_time, iter);
+      const int st = imp_newton_step<MAG, RAD, WP, WE, WF>(P, uu00, pp00, g, dt, pp, global_time, iter);
       if (st != IMP_CONT) {
-        total += iter;
+        src.add_iters(c, iter);
         if (st == IMP_OK) {
           double uu[NV];
           p2u<MAG, RAD>(P, pp, uu, g);
-          src.done_ok(c, uu, pp, iter, a, wp, we, wf, total);
+          src.done_ok(c, uu, pp, iter, a, WP, WE, WF);
           n_ok++;
           n_it += iter;
-          active = false;
-        } else if (++a == 6) {
-          src.done_fail(c, total);
+        } else if (a == 5) {
+          src.done_fail(c);
           n_bad++;
-          active = false;
         } else {
-          iter = 0;
-#pragma unroll
-          for (int i = 0; i < NV; i++) pp[i] = pp00[i];
+          push = true;
+          c_push = c;
         }
+        active = false;
       }
     }
   }
@@ -104,6 +168,13 @@ __device__ __forceinline__ void implicit_persistent(const koral_params &P, const
       if (n_bad) atomicAdd(&counts[2], n_bad);
     }
   }
+}
+
+// clears n (<= 64) counters on the launching stream (a kernel, so the clear
+// can sit inside a captured graph)
+static __global__ void k_zero_u32(unsigned *q, int n)
+{
+  if ((int)threadIdx.x < n) q[threadIdx.x] = 0u;
 }
 
 // Blocks of bs threads that fill every CU at the kernel's occupancy (never
@@ -124,8 +195,38 @@ int resident_blocks(K kernel, int bs, long n, unsigned *blocks)
   return 0;
 }
 
-// clears a work counter on the launching stream (a kernel, so the clear can
-// sit inside a captured graph)
-static __global__ void k_zero_u32(unsigned int *q) { *q = 0u; }
+template <int MAG, int WP, int WE, int WF, class Src>
+int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, double gt, int *counts,
+                    hipStream_t s)
+{
+  unsigned blocks = 0;
+  int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, Src>, 256, n, &blocks);
+  if (rc) return rc;
+  k_imp_pass<MAG, 1, WP, WE, WF, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts);
+  return 0;
+}
+
+// all six passes for n cells (asynchronous on s); counts may be null
+template <int MAG, class Src>
+int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, double gt, int *counts,
+                    hipStream_t s)
+{
+  k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
+  k_imp_classify<MAG, 1, Src><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, src, n, b);
+  int rc = 0;
+  for (int a = 0; a < 6 && !rc; a++) {
+    if (a >= 4) {
+      rc = imp_launch_pass<MAG, P_MHD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+    } else if (a == 1 || a == 3) {
+      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, FF>(P, src, n, a, b, gt, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, FF>(P, src, n, a, b, gt, counts, s);
+    } else {
+      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+    }
+  }
+  return rc;
+}
 
 }  // namespace kd

@@ -6,6 +6,7 @@
 #include <string.h>
 #include "koral_hip.h"
 #include <mutex>
+#include <vector>
 #include "imp_persist.hpp"
 #include "capi_util.hpp"
 
@@ -133,7 +134,8 @@ __global__ void __launch_bounds__(256) k_Gi(koral_params P, long n, const double
   for (int k = 0; k < 4; k++) Gi[k * n + i] = G4[k];
 }
 
-// cell source of the batched implicit solve: SoA arrays with stride n
+// cell source of the batched implicit solve: SoA arrays with stride n;
+// info[5] accumulates the iterations of all attempts
 template <int MAG, int RAD>
 struct BatchCells {
   long n;
@@ -151,8 +153,10 @@ struct BatchCells {
     }
     dtc = dt[c];
   }
+  __device__ __forceinline__ void init(long c) const { info[5 * n + c] = 0; }
+  __device__ __forceinline__ void add_iters(long c, int it) const { info[5 * n + c] += it; }
   __device__ __forceinline__ void done_ok(long c, const double *u, const double *p, int iters, int attempt, int wp,
-                                          int we, int wf, int total) const
+                                          int we, int wf) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
 #pragma unroll
@@ -168,9 +172,8 @@ struct BatchCells {
     info[2 * n + c] = wp;
     info[3 * n + c] = we;
     info[4 * n + c] = wf;
-    info[5 * n + c] = total;
   }
-  __device__ __forceinline__ void done_fail(long c, int total) const
+  __device__ __forceinline__ void done_fail(long c) const
   {
     flags[F_RADSOURCETYPE * n + c] = 3;
     flags[F_RADFIXUP * n + c] = -1;
@@ -180,47 +183,71 @@ struct BatchCells {
     info[2 * n + c] = 0;
     info[3 * n + c] = 0;
     info[4 * n + c] = 0;
-    info[5 * n + c] = total;
   }
 };
 
-constexpr int IMP_BS = 256;
-
-template <int MAG, int RAD>
-__global__ void __launch_bounds__(IMP_BS) k_implicit(koral_params P, BatchCells<MAG, RAD> src,
-                                                     unsigned int *queue, double global_time)
-{
-  implicit_persistent<MAG, RAD>(P, src, src.n, queue, global_time, nullptr);
-}
-
-// Work counters of the persistent implicit launch: one allocation per device
-// (made on the first call), a ring of slots so calls on different streams
-// never share a counter; each call clears its slot with a kernel on its
-// stream before the launch.
-constexpr int QRING = 1024;
-struct QueueRing {
-  unsigned int *slots = nullptr;
-  unsigned int next = 0;
+// Work lists and counters of the implicit passes, per device.  A set is
+// reused by a call on the stream of its last call (stream order separates
+// them) or, on another stream, once the work of its last call has finished
+// (an event recorded after the launches), so overlapping calls never share
+// one; the sets grow to the largest batch seen and live until the process
+// ends.
+struct ImpSet {
+  int dev = 0;
+  long cap = 0;
+  bool busy = false;
+  hipStream_t last = nullptr;
+  ImpBufs b{};
+  void *mem = nullptr;
+  hipEvent_t done = nullptr;
 };
-std::mutex qmu;
-QueueRing qrings[64];
+std::mutex pool_mu;
+std::vector<ImpSet *> pool;
 
-int queue_slot(unsigned int **out)
+int acquire_set(long n, hipStream_t st, ImpSet **out)
 {
   int dev = 0;
   int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
   if (rc) return rc;
-  if (dev < 0 || dev >= 64) return set_error(KORAL_E_ARG, "device index out of range");
-  std::lock_guard<std::mutex> lk(qmu);
-  QueueRing &r = qrings[dev];
-  if (!r.slots) {
-    rc = check_hip(hipMalloc(&r.slots, QRING * sizeof(unsigned int)), "hipMalloc queue");
-    if (rc) return rc;
+  std::lock_guard<std::mutex> lk(pool_mu);
+  for (ImpSet *s : pool)
+    if (!s->busy && s->dev == dev && s->cap >= n && (s->last == st || hipEventQuery(s->done) == hipSuccess)) {
+      s->busy = true;
+      *out = s;
+      return 0;
+    }
+  long cap = n < (1L << 16) ? (1L << 16) : n;
+  auto *s = new ImpSet();
+  s->dev = dev;
+  s->cap = cap;
+  size_t bytes = (4 * (size_t)cap + IMP_NCTR) * sizeof(unsigned);
+  rc = check_hip(hipMalloc(&s->mem, bytes), "hipMalloc implicit lists");
+  if (!rc) rc = check_hip(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "event");
+  if (rc) {
+    if (s->mem) (void)hipFree(s->mem);
+    delete s;
+    return rc;
   }
-  *out = r.slots + (r.next++ % QRING);
+  unsigned *m = (unsigned *)s->mem;
+  s->b.list[0][0] = m;
+  s->b.list[0][1] = m + cap;
+  s->b.list[1][0] = m + 2 * cap;
+  s->b.list[1][1] = m + 3 * cap;
+  s->b.ctr = m + 4 * cap;
+  s->busy = true;
+  pool.push_back(s);
+  *out = s;
   return 0;
 }
 
+int release_set(ImpSet *s, hipStream_t st)
+{
+  int rc = check_hip(hipEventRecord(s->done, st), "event record");
+  std::lock_guard<std::mutex> lk(pool_mu);
+  s->last = st;
+  s->busy = false;
+  return rc;
+}
 
 inline dim3 grid_for(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
@@ -303,22 +330,19 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   if (n == 0) return 0;
   if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "implicit batch larger than 2^31 cells");
   hipStream_t s = (hipStream_t)stream;
-  unsigned int *q = nullptr;
-  int rc = queue_slot(&q);
+  ImpSet *set = nullptr;
+  int rc = acquire_set(n, s, &set);
   if (rc) return rc;
-  k_zero_u32<<<1, 1, 0, s>>>(q);
-  unsigned blocks = 0;
   if (P->mag) {
-    rc = resident_blocks(k_implicit<1, 1>, IMP_BS, n, &blocks);
-    if (rc) return rc;
     BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-    k_implicit<1, 1><<<blocks, IMP_BS, 0, s>>>(*P, src, q, global_time);
+    rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s);
   } else {
-    rc = resident_blocks(k_implicit<0, 1>, IMP_BS, n, &blocks);
-    if (rc) return rc;
     BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-    k_implicit<0, 1><<<blocks, IMP_BS, 0, s>>>(*P, src, q, global_time);
+    rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s);
   }
+  int rc2 = release_set(set, s);
+  if (rc) return rc;
+  if (rc2) return rc2;
   return check_launch("koral_hip_implicit_batch");
 }
 

@@ -93,63 +93,57 @@ struct Timing {
   }
 };
 
-// cell source of the persistent implicit sweep: interior cells in (x, y, z)
-// order of the work index
+// cell source of the implicit passes: interior cells in (x, y, z) order of
+// the index
 template <int MAG>
 struct GridCells {
   DGrid G;
   double *u, *p;
   signed char *fl;
   double dt;
-  __device__ __forceinline__ long cell(long i) const
+  __device__ __forceinline__ long cell(long i, int &ix, int &iy) const
   {
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
-    int ix, iy, iz;
+    int iz;
     b.at(i, ix, iy, iz);
     return cidx(G, ix, iy, iz);
   }
   __device__ __forceinline__ void load(long i, double *uu00, double *pp00, Geom &g, double &dtc) const
   {
-    Box b{0, 0, 0, G.nx, G.ny, G.nz};
-    int ix, iy, iz;
-    b.at(i, ix, iy, iz);
-    const long c = cidx(G, ix, iy, iz);
+    int ix, iy;
+    const long c = cell(i, ix, iy);
     geom_cell(G, ix, iy, g);
     load_p<MAG, 1>(p, G.ncell, c, pp00);
     load_p<MAG, 1>(u, G.ncell, c, uu00);
     dtc = dt;
   }
-  __device__ __forceinline__ void done_ok(long i, const double *uu, const double *pp, int, int, int, int, int,
+  __device__ __forceinline__ void init(long) const {}
+  __device__ __forceinline__ void add_iters(long, int) const {}
+  __device__ __forceinline__ void done_ok(long i, const double *uu, const double *pp, int, int, int, int,
                                           int) const
   {
-    const long c = cell(i), n = G.ncell;
+    int ix, iy;
+    const long c = cell(i, ix, iy), n = G.ncell;
     store_p<MAG, 1>(p, n, c, pp);
     store_p<MAG, 1>(u, n, c, uu);
     fl[F_RADSOURCETYPE * n + c] = 3;
     fl[F_RADFIXUP * n + c] = 0;
     fl[F_RADIMPFIXUP * n + c] = 0;
   }
-  __device__ __forceinline__ void done_fail(long i, int) const
+  __device__ __forceinline__ void done_fail(long i) const
   {
-    const long c = cell(i), n = G.ncell;
+    int ix, iy;
+    const long c = cell(i, ix, iy), n = G.ncell;
     fl[F_RADSOURCETYPE * n + c] = 3;
     fl[F_RADFIXUP * n + c] = -1;
     fl[F_RADIMPFIXUP * n + c] = -1;
   }
 };
 
-template <int MAG>
-__global__ void __launch_bounds__(BS) k_implicit_cells(koral_params P, GridCells<MAG> src, long n,
-                                                       unsigned int *queue, double t, int *counts)
-{
-  implicit_persistent<MAG, 1>(P, src, n, queue, t, counts);
-}
-
 struct HipExec {
   hipStream_t s;
   Timing *tm;
-  unsigned int *queue;
-  unsigned imp_blocks[2];  // resident blocks of k_implicit_cells<MAG>
+  int rc = 0;  // first launch-configuration error
   template <class Op, class L>
   void timed(const Op &op, L launch)
   {
@@ -182,17 +176,15 @@ struct HipExec {
   {
     timed(op, [&](long n) { k_count<Op><<<blocks_for(n), BS, 0, s>>>(op, out); });
   }
-  // the implicit sweep as one persistent launch (lanes refill from a work
-  // counter); the counter is cleared by a kernel on the same stream
+  // the implicit sweep as six per-attempt persistent passes
+  ImpBufs imp{};
   template <int MAG>
   void implicit_cells(const OpImplicit<MAG> &op, int *out)
   {
     timed(op, [&](long n) {
-      const long need = (n + BS - 1) / BS;
-      const unsigned blocks = (unsigned)(need < (long)imp_blocks[MAG] ? need : (long)imp_blocks[MAG]);
-      k_zero_u32<<<1, 1, 0, s>>>(queue);
       GridCells<MAG> src{op.G, op.u, op.p, op.fl, op.dt};
-      k_implicit_cells<MAG><<<blocks, BS, 0, s>>>(op.P, src, n, queue, op.t, out);
+      int r = implicit_passes<MAG>(op.P, src, n, imp, op.t, out, s);
+      if (r && !rc) rc = r;
     });
   }
 };
@@ -258,8 +250,16 @@ struct koral_hip_grid {
   {
     for (void *q : allocs) (void)hipFree(q);
   }
-  unsigned imp_blocks[2] = {1, 1};
-  HipExec ex() { return HipExec{s, &timing, b.queue, {imp_blocks[0], imp_blocks[1]}}; }
+  HipExec ex()
+  {
+    HipExec e{s, &timing};
+    e.imp.list[0][0] = b.imp_list[0][0];
+    e.imp.list[0][1] = b.imp_list[0][1];
+    e.imp.list[1][0] = b.imp_list[1][0];
+    e.imp.list[1][1] = b.imp_list[1][1];
+    e.imp.ctr = b.imp_ctr;
+    return e;
+  }
 };
 
 namespace {
@@ -348,10 +348,10 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   for (int k = 0; k < 7 && !rc; k++) rc = g->alloc(work[k], nv * nc);
   if (!rc) rc = g->alloc(&s.tsd, 1);
   if (!rc) rc = g->alloc(&s.counters, 8);
-  if (!rc) rc = g->alloc(&s.queue, 1);
   const long nd = (long)G.nx * G.ny * G.nz;
-  if (!rc) rc = resident_blocks(k_implicit_cells<0>, BS, nd, &g->imp_blocks[0]);
-  if (!rc) rc = resident_blocks(k_implicit_cells<1>, BS, nd, &g->imp_blocks[1]);
+  for (int i = 0; i < 2 && !rc; i++)
+    for (int k = 0; k < 2 && !rc; k++) rc = g->alloc(&s.imp_list[i][k], nd);
+  if (!rc) rc = g->alloc(&s.imp_ctr, IMP_NCTR);
   if (!rc) rc = g->alloc(&g->stage, nv * nc);
   if (!rc) rc = check_hip(hipMemset(s.p, 0, nv * nc * sizeof(double)), "memset");
   if (!rc) rc = check_hip(hipMemset(s.u, 0, nv * nc * sizeof(double)), "memset");
@@ -544,6 +544,7 @@ int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt)
   if (!g) return set_error(KORAL_E_ARG, "null grid");
   HipExec ex = g->ex();
   implicit_op(ex, g->b, t, dt);
+  if (ex.rc) return ex.rc;
   return check_launch("op_implicit");
 }
 
@@ -556,7 +557,7 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
   if (rc) return rc;
   HipExec ex = g->ex();
   step_rk2imex(ex, g->b, t, dt);
-  rc = check_launch("step_rk2imex");
+  rc = ex.rc ? ex.rc : check_launch("step_rk2imex");
   if (!rc && tstepdenmax) rc = check_hip(hipMemcpyAsync(tstepdenmax, g->b.tsd, sizeof(double), hipMemcpyDeviceToHost, g->s), "tsd");
   if (!rc && counters) rc = check_hip(hipMemcpyAsync(counters, g->b.counters, 6 * sizeof(int), hipMemcpyDeviceToHost, g->s), "counters");
   if (!rc && (tstepdenmax || counters)) rc = check_hip(hipStreamSynchronize(g->s), "sync");
