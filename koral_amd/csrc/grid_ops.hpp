@@ -14,6 +14,7 @@
 // The library's executor launches one HIP kernel per call on the grid's
 // stream (k_grid.hip).
 #pragma once
+#include <type_traits>
 #include "dev_grid.hpp"
 
 namespace kd {
@@ -139,75 +140,113 @@ struct OpBcFaces {
   }
 };
 
-// Ghosts along the four a-b edges of the domain, one thread per (edge,
-// position along the third dimension c): finite.c:3336-3600 recipe (copies
-// of the far edge ghosts, two diagonal averages) in the reference's order.
+// Ghosts along the four a-b edges of the domain, at every position k along
+// the third dimension c: finite.c:3336-3600 recipe, in two launches that keep
+// the reference's data dependences.  Stage A: the copies of the far face
+// ghosts (NG-1 deep on each side of the edge) and the average next to the
+// corner, (ea, eb) = <(ea, ib), (ia, eb)>, all reading face ghosts only.
+// Stage B: the diagonal average (ea2, eb2) = <(ea2, eb), (ea, eb2)> of two
+// stage-A copies.  Each written cell gets u = p2u(p).
 template <int MAG, int RAD>
-struct OpBcEdge {
+struct EdgeGeo {
+  DGrid G;
+  int a, b;
+  KD_HD int c() const { return 3 - a - b; }
+  KD_HD int ngc() const { return c() == 0 ? G.ngx : (c() == 1 ? G.ngy : G.ngz); }  // 0 if collapsed
+  KD_HD int nk() const { return (c() == 0 ? G.nx : (c() == 1 ? G.ny : G.nz)) + 2 * ngc(); }
+  KD_INL long cell(int va, int vb, int k) const
+  {
+    int q[3];
+    q[a] = va;
+    q[b] = vb;
+    q[c()] = k;
+    return cidx(G, q[0], q[1], q[2]);
+  }
+  KD_INL void pos(int va, int vb, int k, int &x, int &y) const
+  {
+    int q[3];
+    q[a] = va;
+    q[b] = vb;
+    q[c()] = k;
+    x = q[0];
+    y = q[1];
+  }
+};
+
+template <int MAG, int RAD>
+KD_INL void edge_set(const koral_params &P, const EdgeGeo<MAG, RAD> &E, double *p, double *u, int k, int da, int db,
+                     int s1a, int s1b, int s2a, int s2b, bool average)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  const long n = E.G.ncell;
+  const long d = E.cell(da, db, k), q1 = E.cell(s1a, s1b, k);
+  double pv[NV];
+  if (average) {
+    const long q2 = E.cell(s2a, s2b, k);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) pv[iv] = .5 * (p[iv * n + q1] + p[iv * n + q2]);
+  } else {
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) pv[iv] = p[iv * n + q1];
+  }
+  store_p<MAG, RAD>(p, n, d, pv);
+  int x, y;
+  E.pos(da, db, k, x, y);
+  Geom g;
+  geom_cell(E.G, x, y, g);
+  double uu[NV];
+  kd::p2u<MAG, RAD>(P, pv, uu, g);
+  store_p<MAG, RAD>(u, n, d, uu);
+}
+
+template <int MAG, int RAD>
+struct OpBcEdgeA {
   static constexpr int kind = K_BC_EDGE;
   koral_params P;
-  DGrid G;
+  EdgeGeo<MAG, RAD> E;
   double *p, *u;
-  int a, b;
-  KD_HD int nk() const
-  {
-    const int c = 3 - a - b;
-    const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
-    const int ngc = c == 0 ? G.ngx : (c == 1 ? G.ngy : G.ngz);  // 0 if collapsed
-    return nc + 2 * ngc;
-  }
-  KD_HD long size() const { return 4L * nk(); }
+  KD_HD int per() const { return 2 * (E.G.ng - 1) + 1; }  // items per (corner, k)
+  KD_HD long size() const { return 4L * E.nk() * per(); }
   KD_INL void operator()(long t) const
   {
-    constexpr int NV = Lay<MAG, RAD>::NV;
-    const int NG = G.ng;
-    const int c = 3 - a - b;
-    const int na = dimn(G, a), nb = dimn(G, b);
-    const int ngc = c == 0 ? G.ngx : (c == 1 ? G.ngy : G.ngz);
-    const int n_k = nk();
-    const int corner = (int)(t / n_k);
-    const int k = -ngc + (int)(t % n_k);
+    const int NG = E.G.ng;
+    const int m = (int)(t % per());
+    const long r = t / per();
+    const int k = -E.ngc() + (int)(r % E.nk());
+    const int corner = (int)(r / E.nk());
     const int ahi = corner >> 1, bhi = corner & 1;
-    const int ea = ahi ? na : -1, ia = ahi ? na - 1 : 0, ea2 = ahi ? na + 1 : -2;
-    const int eb = bhi ? nb : -1, ib = bhi ? nb - 1 : 0, eb2 = bhi ? nb + 1 : -2;
-    auto cell = [&](int va, int vb) {
-      int q[3];
-      q[a] = va;
-      q[b] = vb;
-      q[c] = k;
-      return cidx(G, q[0], q[1], q[2]);
-    };
-    auto fill_u = [&](int va, int vb) {
-      int q[3];
-      q[a] = va;
-      q[b] = vb;
-      q[c] = k;
-      p2u_at<MAG, RAD>(P, G, p, u, q[0], q[1], q[2]);
-    };
-    const long n = G.ncell;
-    for (int i = 0; i < NG - 1; i++) {
-      int ga = ahi ? na + i + 1 : -NG + i;
-      int gb = bhi ? nb + i + 1 : -NG + i;
-      long d1 = cell(ga, eb), s1 = cell(ga, ib), d2 = cell(ea, gb), s2 = cell(ia, gb);
-#pragma unroll
-      for (int iv = 0; iv < NV; iv++) p[iv * n + d1] = p[iv * n + s1];
-#pragma unroll
-      for (int iv = 0; iv < NV; iv++) p[iv * n + d2] = p[iv * n + s2];
-      fill_u(ga, eb);
-      fill_u(ea, gb);
+    const int na = dimn(E.G, E.a), nb = dimn(E.G, E.b);
+    const int ea = ahi ? na : -1, ia = ahi ? na - 1 : 0;
+    const int eb = bhi ? nb : -1, ib = bhi ? nb - 1 : 0;
+    if (m < NG - 1) {
+      const int ga = ahi ? na + m + 1 : -NG + m;
+      edge_set<MAG, RAD>(P, E, p, u, k, ga, eb, ga, ib, 0, 0, false);
+    } else if (m < 2 * (NG - 1)) {
+      const int i = m - (NG - 1);
+      const int gb = bhi ? nb + i + 1 : -NG + i;
+      edge_set<MAG, RAD>(P, E, p, u, k, ea, gb, ia, gb, 0, 0, false);
+    } else {
+      edge_set<MAG, RAD>(P, E, p, u, k, ea, eb, ea, ib, ia, eb, true);
     }
-    {
-      long dd = cell(ea, eb), aa = cell(ea, ib), bb = cell(ia, eb);
-#pragma unroll
-      for (int iv = 0; iv < NV; iv++) p[iv * n + dd] = .5 * (p[iv * n + aa] + p[iv * n + bb]);
-      fill_u(ea, eb);
-    }
-    {
-      long dd = cell(ea2, eb2), aa = cell(ea2, eb), bb = cell(ea, eb2);
-#pragma unroll
-      for (int iv = 0; iv < NV; iv++) p[iv * n + dd] = .5 * (p[iv * n + aa] + p[iv * n + bb]);
-      fill_u(ea2, eb2);
-    }
+  }
+};
+
+template <int MAG, int RAD>
+struct OpBcEdgeB {
+  static constexpr int kind = K_BC_EDGE;
+  koral_params P;
+  EdgeGeo<MAG, RAD> E;
+  double *p, *u;
+  KD_HD long size() const { return 4L * E.nk(); }
+  KD_INL void operator()(long t) const
+  {
+    const int k = -E.ngc() + (int)(t % E.nk());
+    const int corner = (int)(t / E.nk());
+    const int ahi = corner >> 1, bhi = corner & 1;
+    const int na = dimn(E.G, E.a), nb = dimn(E.G, E.b);
+    const int ea = ahi ? na : -1, ea2 = ahi ? na + 1 : -2;
+    const int eb = bhi ? nb : -1, eb2 = bhi ? nb + 1 : -2;
+    edge_set<MAG, RAD>(P, E, p, u, k, ea2, eb2, ea2, eb, ea, eb2, true);
   }
 };
 
@@ -288,51 +327,56 @@ struct OpWavespeeds {
 // per face: reconstruct both sides, face floors, fluxes, Riemann solver
 // (finite.c:694-1014 and f_calc_fluxes_at_faces finite.c:1297-1415)
 // ---------------------------------------------------------------------------
-template <int MAG, int RAD, int ORDER>
-KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__restrict__ p, int d, int ix, int iy,
-                       int iz, double *pL, double *pR)
+// stride between neighbouring cells along the swept direction D (never a
+// collapsed one) and the coordinate index along it
+template <int D>
+KD_INL long dir_stride(const DGrid &G)
+{
+  return D == 0 ? 1L : (D == 1 ? (long)G.sx : (long)G.sx * G.sy);
+}
+template <int D>
+KD_INL int dir_index(int ix, int iy, int iz)
+{
+  return D == 0 ? ix : (D == 1 ? iy : iz);
+}
+
+// Face states of the face between cells -1 and 0 of direction D.  Cells are
+// addressed as offsets from the face's right cell, so no coordinate array is
+// indexed by the direction at run time.
+template <int MAG, int RAD, int ORDER, int D>
+KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__restrict__ p, int ix, int iy, int iz,
+                       double *pL, double *pR)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const long n = G.ncell;
+  const long c0 = cidx(G, ix, iy, iz), st = dir_stride<D>(G);
   if (ORDER == 1) {
-    long c[4];
-#pragma unroll
-    for (int o = 0; o < 4; o++) {
-      int q[3] = {ix, iy, iz};
-      q[d] += o - 2;
-      c[o] = cidx(G, q[0], q[1], q[2]);
-    }
     const double th = P.minmod_theta;
 #pragma unroll
     for (int iv = 0; iv < NV; iv++) {
-      double um1 = p[iv * n + c[0]], u0 = p[iv * n + c[1]], up1 = p[iv * n + c[2]];
+      const double *q = p + iv * n + c0;
+      double um1 = q[-2 * st], u0 = q[-st], up1 = q[0];
       pL[iv] = u0 + .5 * minmod3(th * (up1 - u0), .5 * (up1 - um1), th * (u0 - um1));
-      double vm1 = u0, v0 = up1, vp1 = p[iv * n + c[3]];
+      double vm1 = u0, v0 = up1, vp1 = q[st];
       pR[iv] = v0 - .5 * minmod3(th * (vp1 - v0), .5 * (vp1 - vm1), th * (v0 - vm1));
     }
     return;
   }
-  // cells -3..+2 relative to the face (face between cells -1 and 0)
-  long c[6];
+  // cells -3..+2 relative to the face
   double dx[6];
+  const int id = dir_index<D>(ix, iy, iz);
 #pragma unroll
-  for (int o = 0; o < 6; o++) {
-    int q[3] = {ix, iy, iz};
-    q[d] += o - 3;
-    c[o] = cidx(G, q[0], q[1], q[2]);
-    dx[o] = (ORDER == 2) ? size_x(G, q[d], d) : 0.;
-  }
+  for (int o = 0; o < 6; o++) dx[o] = (ORDER == 2) ? size_x(G, id + o - 3, D) : 0.;
 #pragma unroll
   for (int iv = 0; iv < NV; iv++) {
-    if (ORDER == 4) {
-      double v0 = p[iv * n + c[0]], v1 = p[iv * n + c[1]], v2 = p[iv * n + c[2]], v3 = p[iv * n + c[3]],
-             v4 = p[iv * n + c[4]], v5 = p[iv * n + c[5]];
-      pL[iv] = mp5(v0, v1, v2, v3, v4);  // right face of cell -1
-      pR[iv] = mp5(v5, v4, v3, v2, v1);  // left face of cell 0
-    } else {
-      double v[6];
+    const double *q = p + iv * n + c0;
+    double v[6];
 #pragma unroll
-      for (int o = 0; o < 6; o++) v[o] = p[iv * n + c[o]];
+    for (int o = 0; o < 6; o++) v[o] = q[(o - 3) * st];
+    if (ORDER == 4) {
+      pL[iv] = mp5(v[0], v[1], v[2], v[3], v[4]);  // right face of cell -1
+      pR[iv] = mp5(v[5], v[4], v[3], v[2], v[1]);  // left face of cell 0
+    } else {
       double ul, ur;
       ppm(v, dx, ul, ur);
       pL[iv] = ur;
@@ -342,14 +386,13 @@ KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__re
   }
 }
 
-template <int MAG, int RAD, int ORDER>
+template <int MAG, int RAD, int ORDER, int D>
 struct OpFlux {
   static constexpr int kind = K_FLUX;
   koral_params P;
   DGrid G;
   const double *p, *ws;
   double *flb;
-  int d;
   Box b;
   KD_HD long size() const { return b.size(); }
   KD_INL void operator()(long i) const
@@ -359,28 +402,26 @@ struct OpFlux {
     int ix, iy, iz;
     b.at(i, ix, iy, iz);
     double pL[NV], pR[NV];
-    recon_face<MAG, RAD, ORDER>(P, G, p, d, ix, iy, iz, pL, pR);
+    recon_face<MAG, RAD, ORDER, D>(P, G, p, ix, iy, iz, pL, pR);
     Geom g;
-    geom_face(G, d, ix, iy, g);
+    geom_face(G, D, ix, iy, g);
     floors_mhd<MAG, RAD>(P, pL, g);
     floors_mhd<MAG, RAD>(P, pR, g);
     double fL[NV], fR[NV], uL[NV], uR[NV];
-    flux_prime<MAG, RAD>(P, pL, d + 1, g, fL);
-    flux_prime<MAG, RAD>(P, pR, d + 1, g, fR);
+    flux_prime<MAG, RAD>(P, pL, D + 1, g, fL);
+    flux_prime<MAG, RAD>(P, pR, D + 1, g, fR);
     kd::p2u<MAG, RAD>(P, pL, uL, g);
     kd::p2u<MAG, RAD>(P, pR, uR, g);
-    int qm[3] = {ix, iy, iz};
-    qm[d] -= 1;
-    const long cR = cidx(G, ix, iy, iz), cL = cidx(G, qm[0], qm[1], qm[2]);
     const long n = G.ncell;
-    const long f = fidx(G, d, ix, iy, iz);
-    const long nf = d == 0 ? G.nfx : (d == 1 ? G.nfy : G.nfz);
+    const long cR = cidx(G, ix, iy, iz), cL = cR - dir_stride<D>(G);
+    const long f = fidx(G, D, ix, iy, iz);
+    const long nf = D == 0 ? G.nfx : (D == 1 ? G.nfy : G.nfz);
 #pragma unroll
     for (int iv = 0; iv < NV; iv++) {
       const int r = (RAD && iv >= NVMHD) ? 1 : 0;
-      double ag = mx(ws[(12 + 3 * r + d) * n + cR], ws[(12 + 3 * r + d) * n + cL]);
-      double al = mn(ws[(6 * r + 2 * d) * n + cR], ws[(6 * r + 2 * d) * n + cL]);
-      double ar = mx(ws[(6 * r + 2 * d + 1) * n + cR], ws[(6 * r + 2 * d + 1) * n + cL]);
+      double ag = mx(ws[(12 + 3 * r + D) * n + cR], ws[(12 + 3 * r + D) * n + cL]);
+      double al = mn(ws[(6 * r + 2 * D) * n + cR], ws[(6 * r + 2 * D) * n + cL]);
+      double ar = mx(ws[(6 * r + 2 * D + 1) * n + cR], ws[(6 * r + 2 * D + 1) * n + cL]);
       double F;
       if (P.fluxmethod == 0) {
         F = .5 * (fR[iv] + fL[iv] - ag * (uR[iv] - uL[iv]));
@@ -809,11 +850,16 @@ void set_bc(Ex &ex, const GridBufs &s)
     using T = decltype(t);
     constexpr int M = T::MAG, R = T::RAD;
     ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u, Box{-G.ngx, -G.ngy, -G.ngz, G.sx, G.sy, G.sz}});
+    auto edges = [&](int a, int b) {
+      EdgeGeo<M, R> E{G, a, b};
+      ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u});
+      ex.each(OpBcEdgeB<M, R>{s.P, E, s.p, s.u});
+    };
     if (G.ny > 1) {
-      ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 0, 1});
+      edges(0, 1);
       if (G.nz > 1) {
-        ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 0, 2});
-        ex.each(OpBcEdge<M, R>{s.P, G, s.p, s.u, 1, 2});
+        edges(0, 2);
+        edges(1, 2);
         ex.each(OpBcTotal<M, R>{s.P, G, s.p, s.u});
       }
     }
@@ -852,12 +898,21 @@ void explicit_op(Ex &ex, const GridBufs &s, double dt)
     dispatch(P, [&](auto t) {
       using T = decltype(t);
       constexpr int M = T::MAG, R = T::RAD;
-      if (P.int_order == 4)
-        ex.each(OpFlux<M, R, 4>{P, G, s.p, s.ws, s.flb[d], d, fb});
-      else if (P.int_order == 2)
-        ex.each(OpFlux<M, R, 2>{P, G, s.p, s.ws, s.flb[d], d, fb});
+      auto sweep = [&](auto dd) {
+        constexpr int D = decltype(dd)::value;
+        if (P.int_order == 4)
+          ex.each(OpFlux<M, R, 4, D>{P, G, s.p, s.ws, s.flb[D], fb});
+        else if (P.int_order == 2)
+          ex.each(OpFlux<M, R, 2, D>{P, G, s.p, s.ws, s.flb[D], fb});
+        else
+          ex.each(OpFlux<M, R, 1, D>{P, G, s.p, s.ws, s.flb[D], fb});
+      };
+      if (d == 0)
+        sweep(std::integral_constant<int, 0>{});
+      else if (d == 1)
+        sweep(std::integral_constant<int, 1>{});
       else
-        ex.each(OpFlux<M, R, 1>{P, G, s.p, s.ws, s.flb[d], d, fb});
+        sweep(std::integral_constant<int, 2>{});
     });
   }
   Flx F{s.flb[0], s.flb[1], s.flb[2], G.nfx, G.nfy, G.nfz};
