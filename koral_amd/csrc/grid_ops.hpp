@@ -389,6 +389,7 @@ KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__re
 template <int MAG, int RAD, int ORDER, int D>
 struct OpFlux {
   static constexpr int kind = K_FLUX;
+  static constexpr int waves_per_simd = ORDER == 4 ? 2 : 1;  // MP5 fits two waves with few spills
   koral_params P;
   DGrid G;
   const double *p, *ws;

@@ -20,8 +20,19 @@ constexpr int BS = 256;
 
 inline dim3 blocks_for(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
 
+// Op::waves_per_simd (default 1) asks the compiler to fit that many waves
+// per SIMD in the register file
+template <class Op, class = void>
+struct WavesPerSimd {
+  static constexpr int value = 1;
+};
 template <class Op>
-__global__ void __launch_bounds__(BS) k_each(Op op)
+struct WavesPerSimd<Op, decltype((void)Op::waves_per_simd, void())> {
+  static constexpr int value = Op::waves_per_simd;
+};
+
+template <class Op>
+__global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_each(Op op)
 {
   long i = (long)blockIdx.x * BS + threadIdx.x;
   if (i < op.size()) op(i);
