@@ -575,6 +575,7 @@ struct OpUpdate {
 template <int MAG, int RAD>
 struct OpU2P {
   static constexpr int kind = K_U2P;
+  static constexpr int waves_per_simd = 2;
   koral_params P;
   DGrid G;
   double *u, *p;

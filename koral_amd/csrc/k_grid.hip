@@ -46,7 +46,7 @@ __device__ __forceinline__ void atomic_max_pos(double *addr, double v)
 
 // max over threads: wave shuffle, block via LDS, one atomic per block
 template <class Op>
-__global__ void __launch_bounds__(BS) k_max(Op op, double *out)
+__global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_max(Op op, double *out)
 {
   __shared__ double red[BS / 64];
   long i = (long)blockIdx.x * BS + threadIdx.x;
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(BS) k_max(Op op, double *out)
 
 // three counters: wave sums, one atomic per wave and counter
 template <class Op>
-__global__ void __launch_bounds__(BS) k_count(Op op, int *out)
+__global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_count(Op op, int *out)
 {
   long i = (long)blockIdx.x * BS + threadIdx.x;
   Cnt3 c{{0, 0, 0}};
