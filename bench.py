@@ -112,6 +112,156 @@ def bench_implicit(args):
     return res
 
 
+def _timed_solves(solve, reset, steps, warmup):
+    """Times `steps` solves of one batch (state restored before each by
+    `reset`, outside the event pair).  Returns (wall s per solve, event s
+    per solve, last (flags, info))."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        reset()
+        solve()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kt, wall, out = [], 0.0, None
+    for _ in range(steps):
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        out = solve()
+        ev1.record(stream)
+        ev1.synchronize()
+        wall += time.perf_counter() - t0
+        kt.append(ev0.elapsed_time(ev1) / 1e3)
+    return wall / steps, sum(kt) / len(kt), out
+
+
+def bench_learned(args):
+    """Learned-guess implicit solve: train the network on pairs from the plain
+    FP64 solver over a seeded training batch, then time the plain and the
+    learned solve of a fresh batch of the same generator."""
+    import numpy as np
+    import torch
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd import learn as LN
+
+    P = KP.params_implicit_batch()
+    dev = torch.device("cuda:0")
+    say = lambda *a: print("[learned]", *a, file=sys.stderr, flush=True)
+    # ---- training pairs from the plain solver ----
+    t0 = time.perf_counter()
+    pp, g20, G20, dt = PR.implicit_batch_states(P, args.train_cells, seed=args.seed + 1000)
+    Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    del pp, g20, G20, dt
+    U = cell.p2u(P, Pp, g, G)
+    P0 = Pp.clone()
+    _, info = cell.implicit(P, U, Pp, g, G, dtd)
+    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
+    del U, Pp, P0, g, G, dtd, info
+    torch.cuda.empty_cache()
+    norm = LN.normalisation(F, T)
+    Fs, Ts = LN.standardise(F, T, norm)
+    del F, T
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    say("%d training pairs from %d cells in %.1f s" % (Fs.shape[1], args.train_cells, t_pairs))
+    net = LN.Net(seed=0)
+    net.set(LN.init_params(0), norm)
+    t0 = time.perf_counter()
+    hist = net.fit(Fs, Ts, epochs=args.epochs, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    say("trained %d epochs in %.1f s, loss %s" % (args.epochs, t_train, hist))
+    n_pairs = Fs.shape[1]
+    del Fs, Ts
+    torch.cuda.empty_cache()
+    # ---- fresh evaluation batch ----
+    n = args.cells
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=args.seed)
+    Pp0, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    U0 = cell.p2u(P, Pp0, g, G)
+    U, Pp = torch.empty_like(U0), torch.empty_like(Pp0)
+
+    def reset():
+        U.copy_(U0)
+        Pp.copy_(Pp0)
+
+    w_plain, k_plain, (_, ia) = _timed_solves(lambda: cell.implicit(P, U, Pp, g, G, dtd), reset,
+                                              args.steps, args.warmup)
+    pa = Pp.clone()
+    say("plain solve %.2f ms" % (w_plain * 1e3))
+    w_learn, k_learn, (_, ib) = _timed_solves(lambda: LN.implicit_learned(P, net, U, Pp, g, G, dtd), reset,
+                                              args.steps, args.warmup)
+    pb = Pp.clone()
+    # network stages alone (features, BF16 inference, guess) on the same batch
+    reset()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    Fe = net.features(P, Pp, dtd)
+    ev[1].record(stream)
+    Ye = net.infer(Fe)
+    ev[2].record(stream)
+    net.guess(Pp, Ye)
+    ev[3].record(stream)
+    ev[3].synchronize()
+    t_feat, t_inf, t_guess = (ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3))
+    ia, ib = ia.cpu().numpy(), ib.cpu().numpy()
+    ok_a, ok_b = ia[1] >= 0, ib[1] >= 0
+    pa, pb = pa.cpu().numpy(), pb.cpu().numpy()
+    both = (ib[1] == 6) & (ia[1] == 0) & (ia[2] == 2)
+    du = np.abs(pa[1] - pb[1]) / np.abs(pa[1])
+    it_a, it_b = float(ia[5].mean()), float(ib[5].mean())
+    # algorithmic flops: the counted FP64 operations of the plain solve scaled
+    # per Newton iteration to the learned solve's iterations, plus the network
+    flops_it = IMPLICIT_FLOPS_PER_SOLVE / it_a
+    mlp_flops = 2.0 * (16 * 128 + 2 * 128 * 128 + 128 * 16)   # padded MFMA tiles
+    learned_flops = flops_it * it_b + mlp_flops
+    achieved = learned_flops * n / k_learn / 1e12
+    inf_tf = mlp_flops * n / t_inf / 1e12
+    res = {
+        "metric": METRIC,
+        "value": n / w_learn,
+        "unit": "implicit solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": w_learn * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "learned-guess implicit radiation-source solve vs plain guess",
+                   "cells": n, "train_cells": args.train_cells, "train_pairs": int(n_pairs),
+                   "epochs": args.epochs, "batch": LN.BATCH, "lr": 1e-3,
+                   "train_loss_per_epoch": hist, "pairs_s": t_pairs, "train_s": t_train,
+                   "plain_solves_per_s": n / w_plain, "learned_solves_per_s": n / w_learn,
+                   "speedup": w_plain / w_learn,
+                   "mean_total_iters_plain": it_a, "mean_total_iters_learned": it_b,
+                   "iteration_ratio": it_a / it_b,
+                   "solved_frac_plain": float(ok_a.mean()), "solved_frac_learned": float(ok_b.mean()),
+                   "learned_attempt_frac": float((ib[1] == 6).mean()),
+                   "same_system_cells": int(both.sum()),
+                   "same_system_du_median": float(np.median(du[both])) if both.any() else None,
+                   "same_system_du_max": float(du[both].max()) if both.any() else None,
+                   "network_ms": {"features": t_feat * 1e3, "infer_bf16": t_inf * 1e3,
+                                  "guess": t_guess * 1e3}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "learned implicit solve: features + BF16 inference + guess + classify + "
+                               "13 k_imp_pass launches (one batch)",
+                     "kernel_ms": k_learn * 1e3,
+                     "algorithmic_flops_per_launch": learned_flops * n,
+                     "flops_basis": "counted FP64 operations per plain solve / plain mean iterations x "
+                                    "learned mean iterations + network MACs x 2",
+                     "infer": {"achieved_TFLOPs": inf_tf, "peak_TFLOPs": 2500.0, "frac": inf_tf / 2500.0,
+                               "bound": "mfma (bf16)"}},
+    }
+    return res
+
+
 def cpu_baseline_implicit(P, pp, g20, G20, dt, args):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -239,11 +389,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="implicit", choices=["implicit", "grmhd", "grrmhd"])
+    ap.add_argument("--workload", default="implicit", choices=["implicit", "learned", "grmhd", "grrmhd"])
     ap.add_argument("--grid", type=int, nargs=3, default=None)
     ap.add_argument("--cpu-grid", type=int, nargs=3, default=[48, 48, 24])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cells", type=int, default=1 << 22)
+    ap.add_argument("--train-cells", type=int, default=1 << 23)
+    ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -253,7 +405,12 @@ def main():
         print("bench.py runs on one GPU only: the single-GPU build holds the whole domain "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
-    res = bench_implicit(args) if args.workload == "implicit" else bench_grid(args, args.workload)
+    if args.workload == "implicit":
+        res = bench_implicit(args)
+    elif args.workload == "learned":
+        res = bench_learned(args)
+    else:
+        res = bench_grid(args, args.workload)
     print(json.dumps(res), flush=True)
 
 

@@ -15,6 +15,13 @@
 // scheme, to the list of the next attempt.  Cells need 1 to 51 iterations
 // per attempt, so no lane waits for the slowest cell of its wave.
 //
+// With a learned initial guess (gx != null) a first pass runs before the
+// chain: the energy-equation lab-frame solve on MHD primitives from the guess
+// (u, u~^1..3 replaced by gx[0..3][c]) over all cells; a cell it does not
+// solve within the iteration budget enters attempt 0 of the plain chain
+// unchanged, so the guess can change the iteration count but not what the
+// chain would do without it.  A cell solved by that pass reports attempt 6.
+//
 // Src provides, for a cell index c:
 //   load(c, uu00, pp00, g, dt)     inputs of the solve
 //   init(c)                        before the first attempt
@@ -30,8 +37,10 @@ namespace kd {
 // Device buffers of the passes: two list pairs (ping-pong, each n entries per
 // primitive kind) and IMP_NCTR counters.  Counter 4 a + k: k = 0, 1 the list
 // lengths for attempt a (MHD, RAD); k = 2, 3 the work counters of its two
-// kernels.
+// kernels; counter IMP_LEARN_Q the work counter of the learned-guess pass.
 constexpr int IMP_NCTR = 32;
+constexpr int IMP_LEARN_Q = 28;
+constexpr int IMP_LEARNED_ATTEMPT = 6;
 struct ImpBufs {
   unsigned *list[2][2];
   unsigned *ctr;
@@ -63,18 +72,21 @@ __device__ __forceinline__ void wave_append(bool pred, unsigned v, unsigned *lis
 
 // sorts the n cells into the two attempt-0 lists by the primitives the
 // chain starts with
+// (without lists, for the learned pass, only the per-cell init)
 template <int MAG, int RAD, class Src>
-__global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, long n, ImpBufs b)
+__global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, long n, ImpBufs b, int lists)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int wp = 0;
   if (c < n) {
-    double uu00[NV], pp00[NV], dt;
-    Geom g;
-    src.load(c, uu00, pp00, g, dt);
-    wp = imp_startwith<MAG, RAD>(P, pp00, g);
     src.init(c);
+    if (lists) {
+      double uu00[NV], pp00[NV], dt;
+      Geom g;
+      src.load(c, uu00, pp00, g, dt);
+      wp = imp_startwith<MAG, RAD>(P, pp00, g);
+    }
   }
   wave_append(wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
   wave_append(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
@@ -82,19 +94,22 @@ __global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, l
 
 // attempt a on the cells of one list, all iterating on primitives WP with
 // equation WE in frame WF
-template <int MAG, int RAD, int WP, int WE, int WF, class Src>
+template <int MAG, int RAD, int WP, int WE, int WF, bool LEARN, class Src>
 __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, double global_time,
-                                                  int *counts)
+                                                  int *counts, const double *gx, long n_all)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const int lane = threadIdx.x & 63;
   const int kind = (WP == P_MHD) ? 0 : 1;
-  const unsigned *list = b.list[a & 1][kind];
-  unsigned *out0 = b.list[(a + 1) & 1][0], *out1 = b.list[(a + 1) & 1][1];
-  const long n = b.ctr[4 * a + kind];
-  unsigned *queue = &b.ctr[4 * a + 2 + kind];
-  unsigned *out_len = &b.ctr[4 * (a + 1)];
-  const int next_kind = (imp_next_wp(a, WP) == P_MHD) ? 0 : 1;
+  // the learned pass takes every cell in index order and hands failures to
+  // attempt 0; attempt a takes its list and hands failures to attempt a + 1
+  const unsigned *list = LEARN ? nullptr : b.list[a & 1][kind];
+  unsigned *out0 = LEARN ? b.list[0][0] : b.list[(a + 1) & 1][0];
+  unsigned *out1 = LEARN ? b.list[0][1] : b.list[(a + 1) & 1][1];
+  const long n = LEARN ? n_all : (long)b.ctr[4 * a + kind];
+  unsigned *queue = LEARN ? &b.ctr[IMP_LEARN_Q] : &b.ctr[4 * a + 2 + kind];
+  unsigned *out_len = LEARN ? &b.ctr[0] : &b.ctr[4 * (a + 1)];
+  int next_kind = LEARN ? 0 : ((imp_next_wp(a, WP) == P_MHD) ? 0 : 1);
   bool active = false, fresh = false, exhausted = n == 0;
   unsigned c = 0, c_push = 0;
   bool push = false;
@@ -118,7 +133,7 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
         if (!active) {
           const long k = (long)base + __popcll(need & ((1ull << lane) - 1ull));
           if (k < n) {
-            c = list[k];
+            c = LEARN ? (unsigned)k : list[k];
             active = true;
             fresh = true;
           }
@@ -133,6 +148,10 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
       if (fresh) {
 #pragma unroll
         for (int i = 0; i < NV; i++) pp[i] = pp00[i];
+        if (LEARN) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) pp[1 + i] = gx[i * n_all + c];
+        }
         iter = 0;
         fresh = false;
       }
@@ -142,13 +161,14 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
         if (st == IMP_OK) {
           double uu[NV];
           p2u<MAG, RAD>(P, pp, uu, g);
-          src.done_ok(c, uu, pp, iter, a, WP, WE, WF);
+          src.done_ok(c, uu, pp, iter, LEARN ? IMP_LEARNED_ATTEMPT : a, WP, WE, WF);
           n_ok++;
           n_it += iter;
-        } else if (a == 5) {
+        } else if (!LEARN && a == 5) {
           src.done_fail(c);
           n_bad++;
         } else {
+          if (LEARN) next_kind = (imp_startwith<MAG, RAD>(P, pp00, g) == P_MHD) ? 0 : 1;
           push = true;
           c_push = c;
         }
@@ -195,25 +215,27 @@ int resident_blocks(K kernel, int bs, long n, unsigned *blocks)
   return 0;
 }
 
-template <int MAG, int WP, int WE, int WF, class Src>
+template <int MAG, int WP, int WE, int WF, bool LEARN = false, class Src>
 int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s)
+                    hipStream_t s, const double *gx = nullptr)
 {
   unsigned blocks = 0;
-  int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, Src>, 256, n, &blocks);
+  int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src>, 256, n, &blocks);
   if (rc) return rc;
-  k_imp_pass<MAG, 1, WP, WE, WF, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts);
+  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts, gx, n);
   return 0;
 }
 
-// all six passes for n cells (asynchronous on s); counts may be null
+// all passes for n cells (asynchronous on s); counts may be null; gx, when
+// given, holds the learned guess [4][n] of (u, u~^1..3)
 template <int MAG, class Src>
 int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s)
+                    hipStream_t s, const double *gx = nullptr)
 {
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
-  k_imp_classify<MAG, 1, Src><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, src, n, b);
+  k_imp_classify<MAG, 1, Src><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, src, n, b, gx ? 0 : 1);
   int rc = 0;
+  if (gx) rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(P, src, n, 0, b, gt, counts, s, gx);
   for (int a = 0; a < 6 && !rc; a++) {
     if (a >= 4) {
       rc = imp_launch_pass<MAG, P_MHD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);

@@ -199,6 +199,9 @@ struct ImpSet {
   hipStream_t last = nullptr;
   ImpBufs b{};
   void *mem = nullptr;
+  void *lmem = nullptr;  // learned-guess buffers (features, outputs, guess), made on first use
+  float *feat = nullptr, *y = nullptr;
+  double *gx = nullptr;
   hipEvent_t done = nullptr;
 };
 std::mutex pool_mu;
@@ -237,6 +240,19 @@ int acquire_set(long n, hipStream_t st, ImpSet **out)
   s->busy = true;
   pool.push_back(s);
   *out = s;
+  return 0;
+}
+
+int learned_buffers(ImpSet *s)
+{
+  if (s->lmem) return 0;
+  const size_t cap = (size_t)s->cap;
+  int rc = check_hip(hipMalloc(&s->lmem, cap * (11 * sizeof(float) + 4 * sizeof(float) + 4 * sizeof(double))),
+                     "hipMalloc learned buffers");
+  if (rc) return rc;
+  s->gx = (double *)s->lmem;
+  s->feat = (float *)(s->gx + 4 * cap);
+  s->y = s->feat + 11 * cap;
   return 0;
 }
 
@@ -344,6 +360,37 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   if (rc) return rc;
   if (rc2) return rc2;
   return check_launch("koral_hip_implicit_batch");
+}
+
+int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, double *uu, double *pp,
+                                     const double *g20, const double *G20, const double *dt, double global_time,
+                                     int *flags, int *info, void *stream)
+{
+  if (check_params(P) || n < 0 || !m) return set_error(KORAL_E_ARG, "implicit_batch_learned: bad arguments");
+  if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
+  if (n == 0) return 0;
+  if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "implicit batch larger than 2^31 cells");
+  hipStream_t s = (hipStream_t)stream;
+  ImpSet *set = nullptr;
+  int rc = acquire_set(n, s, &set);
+  if (rc) return rc;
+  rc = learned_buffers(set);
+  if (!rc) rc = koral_hip_mlp_features(P, m, n, pp, dt, set->feat, stream);
+  if (!rc) rc = koral_hip_mlp_infer(m, set->feat, n, set->y, stream);
+  if (!rc) rc = koral_hip_mlp_guess(m, n, pp, set->y, set->gx, stream);
+  if (!rc) {
+    if (P->mag) {
+      BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
+      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gx);
+    } else {
+      BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
+      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gx);
+    }
+  }
+  int rc2 = release_set(set, s);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  return check_launch("koral_hip_implicit_batch_learned");
 }
 
 }  // extern "C"

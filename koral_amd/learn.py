@@ -1,0 +1,222 @@
+"""Learned initial guess of the implicit radiation solve: host driver of the
+network kernels in the C-ABI (include/koral_hip.h, koral_hip_mlp_*).
+
+    net = Net(seed=0)                                # parameters on the device
+    X, T = training_pairs(P, pp, g20, G20, dt)       # from the plain solver
+    net.fit(X, T, epochs=4)                          # HIP forward/backward/Adam
+    flags, info = implicit_learned(P, net, uu, pp, g20, G20, dt)
+
+Training pairs come from the plain FP64 solver: the cells it solves with the
+energy equation in the lab frame (attempt 0 or 2 of the fallback chain, the
+same equation on either primitive set) give (features of the initial
+state, targets from the converged state).  Arrays are SoA torch CUDA
+tensors as in koral_amd.cell; torch only holds memory and shuffles batches.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import lib, check
+from .cell import _ptr, _stream, _chk
+
+NIN, NH, NOUT = 11, 128, 4
+BATCH = 4096
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def _sig(L):
+    if getattr(L, "_mlp_sigs", False):
+        return L
+    vp, i, l, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float
+    for name, args in {
+        "koral_hip_mlp_create": [ctypes.POINTER(vp)],
+        "koral_hip_mlp_destroy": [vp],
+        "koral_hip_mlp_set": [vp, vp, vp],
+        "koral_hip_mlp_get": [vp, vp, vp],
+        "koral_hip_mlp_features": [vp, vp, l, vp, vp, vp, vp],
+        "koral_hip_mlp_targets": [vp, l, i, vp, vp, vp, vp],
+        "koral_hip_mlp_forward": [vp, vp, i, vp, vp],
+        "koral_hip_mlp_grad": [vp, vp, vp, i, vp, vp, vp],
+        "koral_hip_mlp_train_step": [vp, vp, vp, i, f, vp, vp],
+        "koral_hip_mlp_infer": [vp, vp, l, vp, vp],
+        "koral_hip_mlp_guess": [vp, l, vp, vp, vp, vp],
+        "koral_hip_implicit_batch_learned": [vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp],
+    }.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i
+    L.koral_hip_mlp_nparam.restype = i
+    L._mlp_sigs = True
+    return L
+
+
+def nparam():
+    return _sig(lib()).koral_hip_mlp_nparam()
+
+
+def init_params(seed=0):
+    """He-style uniform initial weights, zero biases (float32)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for o, i in ((NH, NIN), (NH, NH), (NH, NH), (NOUT, NH)):
+        lim = np.sqrt(6.0 / i)
+        parts.append(rng.uniform(-lim, lim, (o, i)).astype(np.float32).ravel())
+        parts.append(np.zeros(o, np.float32))
+    return np.concatenate(parts)
+
+
+class Net:
+    def __init__(self, seed=0, theta=None, norm=None):
+        self.L = _sig(lib())
+        h = ctypes.c_void_p()
+        check(self.L.koral_hip_mlp_create(ctypes.byref(h)), "mlp_create")
+        self.h = h
+        theta = init_params(seed) if theta is None else np.asarray(theta, np.float32)
+        if norm is None:
+            norm = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)])
+        self.set(theta, norm)
+
+    def close(self):
+        if self.h:
+            self.L.koral_hip_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, theta, norm):
+        th = np.ascontiguousarray(theta, np.float32)
+        nm = np.ascontiguousarray(norm, np.float32)
+        assert th.size == nparam() and nm.size == 2 * (NIN + NOUT)
+        check(self.L.koral_hip_mlp_set(self.h, th.ctypes.data_as(ctypes.c_void_p),
+                                       nm.ctypes.data_as(ctypes.c_void_p)), "mlp_set")
+
+    def get(self):
+        th = np.zeros(nparam(), np.float32)
+        nm = np.zeros(2 * (NIN + NOUT), np.float32)
+        check(self.L.koral_hip_mlp_get(self.h, th.ctypes.data_as(ctypes.c_void_p),
+                                       nm.ctypes.data_as(ctypes.c_void_p)), "mlp_get")
+        return th, nm
+
+    # ---- FP32 training kernels on one batch [11][B], [4][B] ----
+    def forward(self, X):
+        B = X.shape[1]
+        Y = torch.empty((NOUT, B), dtype=torch.float32, device=X.device)
+        check(self.L.koral_hip_mlp_forward(self.h, _ptr(_chk(X, torch.float32)), B, _ptr(Y), _stream()),
+              "mlp_forward")
+        return Y
+
+    def grad(self, X, T):
+        B = X.shape[1]
+        g = torch.empty(nparam(), dtype=torch.float32, device=X.device)
+        loss = ctypes.c_float(0.)
+        check(self.L.koral_hip_mlp_grad(self.h, _ptr(_chk(X, torch.float32)), _ptr(_chk(T, torch.float32)), B,
+                                        _ptr(g), ctypes.byref(loss), _stream()), "mlp_grad")
+        return loss.value, g
+
+    def step(self, X, T, lr=1e-3, want_loss=False):
+        B = X.shape[1]
+        loss = ctypes.c_float(0.)
+        check(self.L.koral_hip_mlp_train_step(self.h, _ptr(_chk(X, torch.float32)), _ptr(_chk(T, torch.float32)),
+                                              B, float(lr), ctypes.byref(loss) if want_loss else None,
+                                              _stream()), "mlp_train_step")
+        return loss.value if want_loss else None
+
+    def fit(self, X, T, epochs=4, lr=1e-3, batch=BATCH, seed=0):
+        """Adam over shuffled mini-batches of the standardised pairs
+        X [11][N], T [4][N] (device).  Returns the mean loss of each epoch
+        (from every 64th batch)."""
+        N = X.shape[1]
+        gen = torch.Generator(device=X.device)
+        gen.manual_seed(seed)
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(N, device=X.device, generator=gen)
+            losses = []
+            for k, b0 in enumerate(range(0, N - batch + 1, batch)):
+                idx = perm[b0:b0 + batch]
+                xb = X[:, idx].contiguous()
+                tb = T[:, idx].contiguous()
+                l = self.step(xb, tb, lr, want_loss=(k % 64 == 0))
+                if l is not None:
+                    losses.append(l)
+            hist.append(float(np.mean(losses)) if losses else float("nan"))
+        return hist
+
+    # ---- inference ----
+    def features(self, P, pp, dt, standardise=True):
+        n = pp.shape[1]
+        F = torch.empty((NIN, n), dtype=torch.float32, device=pp.device)
+        check(self.L.koral_hip_mlp_features(ctypes.byref(P), self.h if standardise else None, n, _ptr(_chk(pp)),
+                                            _ptr(_chk(dt)), _ptr(F), _stream()), "mlp_features")
+        return F
+
+    def infer(self, F):
+        n = F.shape[1]
+        Y = torch.empty((NOUT, n), dtype=torch.float32, device=F.device)
+        check(self.L.koral_hip_mlp_infer(self.h, _ptr(_chk(F, torch.float32)), n, _ptr(Y), _stream()),
+              "mlp_infer")
+        return Y
+
+    def guess(self, pp, Y):
+        n = pp.shape[1]
+        gx = torch.empty((4, n), dtype=torch.float64, device=pp.device)
+        check(self.L.koral_hip_mlp_guess(self.h, n, _ptr(_chk(pp)), _ptr(_chk(Y, torch.float32)), _ptr(gx),
+                                         _stream()), "mlp_guess")
+        return gx
+
+
+def raw_targets(pp0, pp1):
+    L = _sig(lib())
+    n = pp0.shape[1]
+    T = torch.empty((NOUT, n), dtype=torch.float32, device=pp0.device)
+    check(L.koral_hip_mlp_targets(None, n, pp0.shape[0], _ptr(_chk(pp0)), _ptr(_chk(pp1)), _ptr(T), _stream()),
+          "mlp_targets")
+    return T
+
+
+def training_pairs(P, pp0, pp1, dt, info, net=None):
+    """Raw features [11][m] and targets [4][m] of the cells the plain solver
+    solved with the energy equation in the lab frame (attempts 0 and 2)."""
+    sel = (info[1] == 0) | (info[1] == 2)
+    L = _sig(lib())
+    n = pp0.shape[1]
+    F = torch.empty((NIN, n), dtype=torch.float32, device=pp0.device)
+    check(L.koral_hip_mlp_features(ctypes.byref(P), None, n, _ptr(_chk(pp0)), _ptr(_chk(dt)), _ptr(F),
+                                   _stream()), "mlp_features")
+    T = raw_targets(pp0, pp1)
+    keep = sel & torch.isfinite(F).all(0) & torch.isfinite(T).all(0)
+    return F[:, keep].contiguous(), T[:, keep].contiguous()
+
+
+def normalisation(F, T):
+    """[feature mean, std, target mean, std] (float32) of raw pairs."""
+    fm, fs = F.double().mean(1), F.double().std(1)
+    tm, ts = T.double().mean(1), T.double().std(1)
+    fs = torch.where(fs > 0, fs, torch.ones_like(fs))
+    ts = torch.where(ts > 0, ts, torch.ones_like(ts))
+    return torch.cat([fm, fs, tm, ts]).float().cpu().numpy()
+
+
+def standardise(F, T, norm):
+    nm = torch.from_numpy(norm).to(F.device)
+    Fs = ((F - nm[:NIN, None]) / nm[NIN:2 * NIN, None]).contiguous()
+    Ts = ((T - nm[2 * NIN:2 * NIN + NOUT, None]) / nm[2 * NIN + NOUT:, None]).contiguous()
+    return Fs, Ts
+
+
+def implicit_learned(P, net, uu, pp, g20, G20, dt, global_time=0.0):
+    """koral_hip_implicit_batch_learned in place on uu/pp; (flags, info)."""
+    n = pp.shape[1]
+    flags = torch.zeros((7, n), dtype=torch.int32, device=pp.device)
+    info = torch.zeros((6, n), dtype=torch.int32, device=pp.device)
+    check(net.L.koral_hip_implicit_batch_learned(ctypes.byref(P), net.h, n, _ptr(_chk(uu)), _ptr(_chk(pp)),
+                                                 _ptr(_chk(g20)), _ptr(_chk(G20)), _ptr(_chk(dt)),
+                                                 float(global_time), _ptr(flags), _ptr(info), _stream()),
+          "implicit_learned")
+    return flags, info

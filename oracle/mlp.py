@@ -1,0 +1,122 @@
+"""Reference (numpy) of the learned initial-guess network: features, targets,
+forward and backward passes, Adam.  Test infrastructure: it checks the HIP
+training and inference kernels (koral_amd/csrc/k_mlp.hip) and is pinned by
+finite differences in tests/test_oracle_mlp.py.
+
+Network: 11 inputs -> 128 -> 128 -> 128 (GELU, tanh form) -> 4 outputs.
+Parameters are one flat float32 vector in the order W1[128][11], b1[128],
+W2[128][128], b2[128], W3[128][128], b3[128], W4[4][128], b4[4]
+(row-major, W[out][in]).
+
+Features of a cell (standardised with the training set's mean and std):
+  log10 rho, log10(u/rho), log10(E/u), the three VELR components of the gas
+  and of the radiation frame, log10(dt kappa_abs), log10(dt kappa_es)
+Targets (standardised with the training set's std): log(u*/u0) and the
+three differences u~*^i - u~0^i between the converged state of the
+energy-equation lab-frame solve and the initial state.
+"""
+import numpy as np
+
+NIN, NH, NOUT = 11, 128, 4
+SHAPES = [(NH, NIN), (NH,), (NH, NH), (NH,), (NH, NH), (NH,), (NOUT, NH), (NOUT,)]
+NPARAM = sum(int(np.prod(s)) for s in SHAPES)
+C_GELU = 0.7978845608028654  # sqrt(2/pi)
+
+
+def split(theta):
+    out, o = [], 0
+    for s in SHAPES:
+        k = int(np.prod(s))
+        out.append(theta[o:o + k].reshape(s))
+        o += k
+    return out
+
+
+def init_params(seed=0):
+    """He-style uniform initialisation, zero biases (float32)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for s in SHAPES:
+        if len(s) == 2:
+            lim = np.sqrt(6.0 / s[1])
+            parts.append(rng.uniform(-lim, lim, s).astype(np.float32).ravel())
+        else:
+            parts.append(np.zeros(s, np.float32))
+    return np.concatenate(parts)
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + np.tanh(C_GELU * (z + 0.044715 * z ** 3)))
+
+
+def gelu_grad(z):
+    t = np.tanh(C_GELU * (z + 0.044715 * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C_GELU * (1.0 + 3.0 * 0.044715 * z * z)
+
+
+def forward(theta, X):
+    """X [B, 11] -> (Y [B, 4], cache)."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = split(theta)
+    Z1 = X @ W1.T + b1
+    A1 = gelu(Z1)
+    Z2 = A1 @ W2.T + b2
+    A2 = gelu(Z2)
+    Z3 = A2 @ W3.T + b3
+    A3 = gelu(Z3)
+    Y = A3 @ W4.T + b4
+    return Y, (X, Z1, A1, Z2, A2, Z3, A3)
+
+
+def loss_and_grad(theta, X, T):
+    """Mean squared error over batch and outputs, and its gradient."""
+    Y, (X, Z1, A1, Z2, A2, Z3, A3) = forward(theta, X)
+    W1, b1, W2, b2, W3, b3, W4, b4 = split(theta)
+    B = X.shape[0]
+    R = Y - T
+    loss = (R * R).sum() / (B * NOUT)
+    dY = 2.0 * R / (B * NOUT)
+    gW4 = dY.T @ A3
+    gb4 = dY.sum(0)
+    dZ3 = (dY @ W4) * gelu_grad(Z3)
+    gW3 = dZ3.T @ A2
+    gb3 = dZ3.sum(0)
+    dZ2 = (dZ3 @ W3) * gelu_grad(Z2)
+    gW2 = dZ2.T @ A1
+    gb2 = dZ2.sum(0)
+    dZ1 = (dZ2 @ W2) * gelu_grad(Z1)
+    gW1 = dZ1.T @ X
+    gb1 = dZ1.sum(0)
+    g = np.concatenate([a.ravel() for a in (gW1, gb1, gW2, gb2, gW3, gb3, gW4, gb4)])
+    return loss, g
+
+
+def adam(theta, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (step counts from 1); returns new (theta, m, v)."""
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** step)
+    vh = v / (1.0 - b2 ** step)
+    return theta - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def raw_features(P, pp, dt, kappa_abs, kappa_es):
+    """Unstandardised features [n, 11] of primitive states pp [n, nv]."""
+    ee = P.ee
+    rho, u, E = pp[:, 0], pp[:, 1], pp[:, ee]
+    f = np.empty((pp.shape[0], NIN))
+    f[:, 0] = np.log10(rho)
+    f[:, 1] = np.log10(u / rho)
+    f[:, 2] = np.log10(E / u)
+    f[:, 3:6] = pp[:, 2:5]
+    f[:, 6:9] = pp[:, ee + 1:ee + 4]
+    f[:, 9] = np.log10(dt * kappa_abs)
+    f[:, 10] = np.log10(dt * kappa_es)
+    return f
+
+
+def targets(pp0, pp1):
+    """Raw targets [n, 4] from initial and converged primitives."""
+    t = np.empty((pp0.shape[0], NOUT))
+    t[:, 0] = np.log(pp1[:, 1] / pp0[:, 1])
+    t[:, 1:4] = pp1[:, 2:5] - pp0[:, 2:5]
+    return t

@@ -1,0 +1,180 @@
+"""GPU checks of the learned-guess network kernels against the numpy
+reference (oracle/mlp.py): features, FP32 forward, gradient, Adam step, the
+BF16 matrix-core inference (which pins the operand lane layout), and the
+implicit solve with the trained guess against the plain solve."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch.device("cuda:0")
+
+
+def _bf16(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7fff + ((u >> 16) & 1)) & 0xffff0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _rand_net(seed):
+    import mlp as M
+    rng = np.random.default_rng(seed)
+    th = M.init_params(seed)
+    th += rng.normal(scale=0.05, size=th.size).astype(np.float32)  # non-zero biases too
+    return th.astype(np.float32)
+
+
+def test_features_match_reference(dev, oracle):
+    import torch
+    import mlp as M
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd.learn import Net
+    P = KP.params_implicit_batch()
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 4096, seed=7)
+    net = Net(seed=0)
+    F = net.features(P, cell.to_dev(pp), torch.from_numpy(dt).to(dev), standardise=False).cpu().numpy().T
+    ref = M.raw_features(P, pp, dt, PR.kappa_abs_np(P, pp[:, 0], pp[:, 1]), P.kappa_es_coef * pp[:, 0])
+    err = np.abs(F - ref) / (np.abs(ref) + 1e-3)
+    _say("features max rel err %.3e" % err.max())
+    assert err.max() < 1e-5
+
+
+def test_forward_and_gradient_match_reference(dev):
+    import torch
+    import mlp as M
+    from koral_amd.learn import Net
+    rng = np.random.default_rng(2)
+    th = _rand_net(1)
+    net = Net(theta=th)
+    B = 777
+    X = rng.normal(size=(B, M.NIN)).astype(np.float32)
+    T = rng.normal(size=(B, M.NOUT)).astype(np.float32)
+    Xd = torch.from_numpy(X.T.copy()).to(dev)
+    Td = torch.from_numpy(T.T.copy()).to(dev)
+    Y = net.forward(Xd).cpu().numpy().T
+    Yr, _ = M.forward(th.astype(np.float64), X.astype(np.float64))
+    ey = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    loss, g = net.grad(Xd, Td)
+    lr_, gr = M.loss_and_grad(th.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    g = g.cpu().numpy()
+    eg = np.abs(g - gr) / (np.abs(gr) + 1e-3 * np.abs(gr).max())
+    _say("forward max rel %.3e, loss %.6f vs %.6f, grad max rel %.3e" % (ey, loss, lr_, eg.max()))
+    assert ey < 1e-5
+    assert abs(loss - lr_) < 1e-5 * lr_
+    assert eg.max() < 1e-3
+
+
+def test_adam_step_matches_reference(dev):
+    import torch
+    import mlp as M
+    from koral_amd.learn import Net
+    rng = np.random.default_rng(3)
+    th = _rand_net(4)
+    net = Net(theta=th)
+    X = rng.normal(size=(512, M.NIN)).astype(np.float32)
+    T = rng.normal(size=(512, M.NOUT)).astype(np.float32)
+    Xd, Td = torch.from_numpy(X.T.copy()).to(dev), torch.from_numpy(T.T.copy()).to(dev)
+    ref, m, v = th.astype(np.float64), np.zeros(th.size), np.zeros(th.size)
+    for k in range(1, 4):
+        net.step(Xd, Td, lr=1e-3)
+        _, gr = M.loss_and_grad(ref, X.astype(np.float64), T.astype(np.float64))
+        ref, m, v = M.adam(ref, gr, m, v, k, lr=1e-3)
+    got, _ = net.get()
+    err = np.abs(got - ref).max()
+    _say("3 Adam steps: max |theta - ref| %.3e (lr 1e-3)" % err)
+    assert err < 2e-5
+
+
+def test_bf16_inference_matches_reference(dev):
+    """Outputs of the matrix-core inference against a numpy pass with the same
+    BF16 rounding of weights and layer inputs (FP32 sums)."""
+    import torch
+    import mlp as M
+    from koral_amd.learn import Net
+    rng = np.random.default_rng(5)
+    th = _rand_net(6)
+    net = Net(theta=th)
+    n = 1000  # not a multiple of the 64-cell tile
+    X = rng.normal(size=(n, M.NIN)).astype(np.float32)
+    Y = net.infer(torch.from_numpy(X.T.copy()).to(dev)).cpu().numpy().T
+    W1, b1, W2, b2, W3, b3, W4, b4 = M.split(th)
+    h = _bf16(X)
+    for W, b in ((W1, b1), (W2, b2), (W3, b3)):
+        h = _bf16(M.gelu((h.astype(np.float64) @ _bf16(W).T.astype(np.float64) + b)).astype(np.float32))
+    Yr = h.astype(np.float64) @ _bf16(W4).T.astype(np.float64) + b4
+    Yf, _ = M.forward(th.astype(np.float64), X.astype(np.float64))
+    err = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    err32 = np.abs(Y - Yf).max() / np.abs(Yf).max()
+    _say("bf16 inference: max rel vs bf16 reference %.3e, vs fp64 network %.3e" % (err, err32))
+    assert err < 2e-2
+    assert err32 < 5e-2
+
+
+def test_learned_guess_solve(dev, oracle):
+    """Train on pairs from the plain solver, then solve a fresh batch from the
+    learned guess: every cell the plain chain solves is solved, cells solved
+    by both with the energy equation in the lab frame agree, and the mean
+    number of Newton iterations drops."""
+    import torch
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd import learn as LN
+    P = KP.params_implicit_batch()
+    # training pairs
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 17, seed=21)
+    Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    U = cell.p2u(P, Pp, g, G)
+    P0 = Pp.clone()
+    _, info = cell.implicit(P, U, Pp, g, G, dtd)
+    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
+    norm = LN.normalisation(F, T)
+    Fs, Ts = LN.standardise(F, T, norm)
+    net = LN.Net(seed=0)
+    net.set(LN.init_params(0), norm)
+    hist = net.fit(Fs, Ts, epochs=40, lr=1e-3)
+    _say("training pairs %d, loss per epoch %s" % (F.shape[1], ["%.4f" % h for h in hist]))
+    assert hist[-1] < hist[0]
+    # fresh batch: plain and learned
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 15, seed=22)
+    Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    U0 = cell.p2u(P, Pp, g, G)
+    Ua, Pa = U0.clone(), Pp.clone()
+    _, ia = cell.implicit(P, Ua, Pa, g, G, dtd)
+    Ub, Pb = U0.clone(), Pp.clone()
+    _, ib = LN.implicit_learned(P, net, Ub, Pb, g, G, dtd)
+    ia, ib = ia.cpu().numpy(), ib.cpu().numpy()
+    pa, pb = cell.to_host(Pa), cell.to_host(Pb)
+    ok_a, ok_b = ia[1] >= 0, ib[1] >= 0
+    assert np.all(ok_b[ok_a]), "the learned attempt may only add solved cells"
+    learned = ib[1] == 6
+    # cells the plain chain solved with the same system (energy equation, lab
+    # frame, gas primitives) from the old state
+    both = learned & (ia[1] == 0) & (ia[2] == 2)
+    same = (ib[1] == ia[1]) & ok_a
+    du = np.abs(pa[:, 1] - pb[:, 1]) / np.abs(pa[:, 1])
+    dv = np.abs(pa[:, 2:5] - pb[:, 2:5]).max(1) / (1. + np.abs(pa[:, 2:5]).max(1))
+    it_a, it_b = ia[5].mean(), ib[5].mean()
+    _say("cells %d: plain solved %d, learned solved %d (%d by the learned attempt); mean iterations "
+         "plain %.3f learned %.3f (x%.2f)" %
+         (len(ok_a), ok_a.sum(), ok_b.sum(), learned.sum(), it_a, it_b, it_a / it_b))
+    _say("same system from both starts (%d cells): |du|/u median %.2e p99 %.2e max %.2e; "
+         "velocity max %.2e" % (both.sum(), np.median(du[both]), np.quantile(du[both], 0.99), du[both].max(),
+                                dv[both].max()))
+    # cells the learned attempt did not solve follow the plain chain exactly
+    assert np.array_equal(pa[same], pb[same])
+    assert np.array_equal(pa[:, 0], pb[:, 0])  # the implicit step keeps rho
+    assert learned.sum() > 0.5 * len(ok_a)
+    # two converged Newton solves of one system agree to the solver tolerance
+    assert np.quantile(du[both], 0.99) < 1e-4 and np.quantile(dv[both], 0.99) < 1e-4
+    assert it_b < it_a
