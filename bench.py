@@ -191,7 +191,8 @@ def bench_learned(args):
                                               args.steps, args.warmup)
     pa = Pp.clone()
     say("plain solve %.2f ms" % (w_plain * 1e3))
-    w_learn, k_learn, (_, ib) = _timed_solves(lambda: LN.implicit_learned(P, net, U, Pp, g, G, dtd), reset,
+    w_learn, k_learn, (_, ib) = _timed_solves(lambda: LN.implicit_learned(P, net, U, Pp, g, G, dtd,
+                                                                          maxiter=args.learned_maxiter), reset,
                                               args.steps, args.warmup)
     pb = Pp.clone()
     # network stages alone (features, BF16 inference, guess) on the same batch
@@ -236,6 +237,7 @@ def bench_learned(args):
         "config": {"workload": "learned-guess implicit radiation-source solve vs plain guess",
                    "cells": n, "train_cells": args.train_cells, "train_pairs": int(n_pairs),
                    "epochs": args.epochs, "batch": LN.BATCH, "lr": 1e-3,
+                   "learned_maxiter": args.learned_maxiter or P.radimpmaxiter,
                    "train_loss_per_epoch": hist, "pairs_s": t_pairs, "train_s": t_train,
                    "plain_solves_per_s": n / w_plain, "learned_solves_per_s": n / w_learn,
                    "speedup": w_plain / w_learn,
@@ -396,6 +398,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 22)
     ap.add_argument("--train-cells", type=int, default=1 << 23)
     ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--learned-maxiter", type=int, default=0)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -37,7 +37,9 @@ typedef struct koral_params {
   double gamma;                     /* GAMMA */
   double k_boltz, m_proton, sigma_rad, mu_gas; /* ko.h:40-46 in code units */
   double ggg, ccc, masscm;          /* ko.h:9-15 */
-  int kappa_model, pad1;            /* 0: kappa=a*rho; 1: Kramers (cgs coefficient a) */
+  int kappa_model;                  /* 0: kappa=a*rho; 1: Kramers (cgs coefficient a) */
+  int polar_nc;                     /* NCCORRECTPOLAR with CORRECT_POLARAXIS (spherical-like
+                                       coordinates), 0 without */
   double kappa_a_coef, kappa_es_coef;
   double rhofloor, uurhoratiomin, uurhoratiomax, b2rhoratiomax;
   double gammamaxhd, gammamaxrad, eradfloor;
@@ -136,12 +138,13 @@ int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, v
 /* Newton start gx [4][n] (u, u~^1..3) from the outputs (the plain start where not finite/positive) */
 int koral_hip_mlp_guess(koral_hip_mlp *m, long n, const double *pp, const float *Y, double *gx, void *stream);
 /* koral_hip_implicit_batch with a first attempt from the learned guess (energy equation, lab
-   frame, MHD primitives); cells it does not solve run the unchanged six-attempt chain.
-   info[1*n+i] = 6 marks a cell solved by the learned attempt; info[5*n+i] counts the
-   iterations of all attempts. */
+   frame, MHD primitives) under the unchanged convergence tolerance; cells it does not solve
+   within maxiter iterations (RADIMPMAXITER when maxiter <= 0 or larger) run the unchanged
+   six-attempt chain from the plain start.  info[1*n+i] = 6 marks a cell solved by the
+   learned attempt; info[5*n+i] counts the iterations of all attempts. */
 int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, double *uu, double *pp,
                                      const double *g20, const double *G20, const double *dt, double global_time,
-                                     int *flags, int *info, void *stream);
+                                     int maxiter, int *flags, int *info, void *stream);
 
 
 /* ---- grid operators (device-resident state, one GPU holding the whole
@@ -207,9 +210,14 @@ int koral_hip_tstepden(koral_hip_grid *g, double *tstepdenmax);
 int koral_hip_op_explicit(koral_hip_grid *g, double t, double dt, double *tstepdenmax);
 /* calc_u2p (finite.c:502-561) including cell_fixup and set_bc */
 int koral_hip_calc_u2p(koral_hip_grid *g);
+/* do_correct (finite.c:605-636): with koral_params.polar_nc > 0 (CORRECT_POLARAXIS,
+   spherical-like coordinates) correct_polaraxis (finite.c:4924-5060) on the
+   polar_nc cells next to each polar face; nothing otherwise */
+int koral_hip_do_correct(koral_hip_grid *g);
 /* op_implicit (finite.c:1242-1292) including cell_fixup(FIXUP_RADIMP) */
 int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt);
-/* Body of one RK2IMEX step of ko.c:444-570.  tstepdenmax (next dt = 1/it)
+/* Body of one RK2IMEX step of ko.c:444-570, do_correct after every calc_u2p
+   as ko.c:469-565 calls it.  tstepdenmax (next dt = 1/it)
    and counters[6] = {implicit successes, implicit Newton iterations,
    implicit failures, hd fixup requests, rad fixup requests, entropy
    inversions} may be NULL. */

@@ -22,7 +22,7 @@ namespace kd {
 // Operator kinds (per-kind launch timing and names in the C-ABI)
 enum OpKind {
   K_P2U, K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_FLUX, K_EMF, K_CT, K_UPDATE,
-  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_NKINDS
+  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_NKINDS
 };
 
 struct Box {
@@ -956,6 +956,74 @@ void u2p(Ex &ex, const GridBufs &s)
   set_bc(ex, s);
 }
 
+// do_correct (finite.c:605-636) with CORRECT_POLARAXIS: correct_polaraxis
+// for spherical-like coordinates (finite.c:4924-5060).  One item per
+// (ix, iz, face, ic): the cell ic away from the polar face takes the gas and
+// radiation densities and the x1/x3 velocities of the cell nc away, the x2
+// velocities scaled by |x2 - x2_axis| / |x2_src - x2_axis| (cell-centre code
+// coordinates, set_grid finite.c:1630-1660); p and u = p2u(p) are written for
+// every variable but the field.  The items are independent when
+// NY >= 2 nc + 2 (checked at grid creation).
+template <int MAG, int RAD>
+struct OpPolar {
+  static constexpr int kind = K_POLAR;
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  KD_HD long size() const { return 2L * P.polar_nc * G.nx * G.nz; }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    constexpr int EE = Lay<MAG, RAD>::EE;
+    const int nc = P.polar_nc;
+    const int ix = (int)(t % G.nx);
+    long r = t / G.nx;
+    const int iz = (int)(r % G.nz);
+    r /= G.nz;
+    const int side = (int)(r % 2), ic = (int)(r / 2);
+    const int iy = side == 0 ? ic : G.ny - 1 - ic;
+    const int iysrc = side == 0 ? nc : G.ny - 1 - nc;
+    const double *xb = G.xb1 + G.ng;
+    const double thaxis = side == 0 ? xb[0] : xb[G.ny];
+    const double th = .5 * (xb[iy + 1] + xb[iy]), thsrc = .5 * (xb[iysrc + 1] + xb[iysrc]);
+    const double fac = fabs((th - thaxis) / (thsrc - thaxis));
+    const long n = G.ncell, c = cidx(G, ix, iy, iz), cs = cidx(G, ix, iysrc, iz);
+    double pp[NV], uu[NV];
+    load_p<MAG, RAD>(p, n, c, pp);
+    pp[0] = p[0 * n + cs];
+    pp[1] = p[1 * n + cs];
+    pp[5] = p[5 * n + cs];
+    pp[2] = p[2 * n + cs];
+    pp[4] = p[4 * n + cs];
+    pp[3] = fac * p[3 * n + cs];
+    if (RAD) {
+      pp[EE] = p[EE * n + cs];
+      pp[EE + 1] = p[(EE + 1) * n + cs];
+      pp[EE + 3] = p[(EE + 3) * n + cs];
+      pp[EE + 2] = fac * p[(EE + 2) * n + cs];
+    }
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    kd::p2u<MAG, RAD>(P, pp, uu, g);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) {
+      if (MAG && iv >= 6 && iv <= 8) continue;
+      p[iv * n + c] = pp[iv];
+      u[iv * n + c] = uu[iv];
+    }
+  }
+};
+
+template <class Ex>
+void do_correct(Ex &ex, const GridBufs &s)
+{
+  if (s.P.polar_nc <= 0 || s.G.ny < 2) return;
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    ex.each(OpPolar<T::MAG, T::RAD>{s.P, s.G, s.p, s.u});
+  });
+}
+
 template <class Ex>
 void implicit_op(Ex &ex, const GridBufs &s, double t, double dt)
 {
@@ -984,20 +1052,24 @@ void step_rk2imex(Ex &ex, const GridBufs &s, double t, double dt)
   comb(ex, s, s.drt1, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.ut0);
   comb(ex, s, s.ut1, 1., s.u);
   u2p(ex, s);
+  do_correct(ex, s);
   explicit_op(ex, s, dt);
   comb(ex, s, s.dut1, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
   comb(ex, s, s.u, 1., s.ut0, (dt), s.dut1, (dt * (1. - 2. * gamma)), s.drt1);
   comb(ex, s, s.uforget, 1., s.u);
   u2p(ex, s);
+  do_correct(ex, s);
   implicit_op(ex, s, t, gamma * dt);
   comb(ex, s, s.drt2, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.uforget);
   comb(ex, s, s.ut1, 1., s.u);  // ut2
   u2p(ex, s);
+  do_correct(ex, s);
   explicit_op(ex, s, dt);
   comb(ex, s, s.dut2, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
   comb(ex, s, s.u, 1., s.ut0, (dt / 2.), s.dut1, (dt / 2.), s.dut2);
   comb(ex, s, s.u, 1., s.u, (dt / 2.), s.drt1, (dt / 2.), s.drt2);
   u2p(ex, s);
+  do_correct(ex, s);
 }
 
 }  // namespace kd

@@ -227,15 +227,21 @@ int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const 
 }
 
 // all passes for n cells (asynchronous on s); counts may be null; gx, when
-// given, holds the learned guess [4][n] of (u, u~^1..3)
+// given, holds the learned guess [4][n] of (u, u~^1..3), and the learned
+// pass stops a cell after learned_maxiter iterations when that is positive
+// and below RADIMPMAXITER
 template <int MAG, class Src>
 int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s, const double *gx = nullptr)
+                    hipStream_t s, const double *gx = nullptr, int learned_maxiter = 0)
 {
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
   k_imp_classify<MAG, 1, Src><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, src, n, b, gx ? 0 : 1);
   int rc = 0;
-  if (gx) rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(P, src, n, 0, b, gt, counts, s, gx);
+  if (gx) {
+    koral_params Pl = P;
+    if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
+    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gx);
+  }
   for (int a = 0; a < 6 && !rc; a++) {
     if (a >= 4) {
       rc = imp_launch_pass<MAG, P_MHD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);

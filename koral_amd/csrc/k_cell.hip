@@ -364,7 +364,7 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
 
 int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, double *uu, double *pp,
                                      const double *g20, const double *G20, const double *dt, double global_time,
-                                     int *flags, int *info, void *stream)
+                                     int maxiter, int *flags, int *info, void *stream)
 {
   if (check_params(P) || n < 0 || !m) return set_error(KORAL_E_ARG, "implicit_batch_learned: bad arguments");
   if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
@@ -381,10 +381,10 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, lo
   if (!rc) {
     if (P->mag) {
       BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gx);
+      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gx, maxiter);
     } else {
       BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gx);
+      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gx, maxiter);
     }
   }
   int rc2 = release_set(set, s);

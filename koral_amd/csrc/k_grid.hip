@@ -315,6 +315,8 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   if ((P->int_order == 4 && D->ng < 4) || (P->int_order == 2 && D->ng < 3))
     return set_error(KORAL_E_ARG, "grid_create: NG too small for INT_ORDER");
   if (D->ny == 1 && D->nz > 1) return set_error(KORAL_E_ARG, "grid_create: NY=1 with NZ>1 not supported");
+  if (P->polar_nc < 0 || (P->polar_nc > 0 && D->ny > 1 && D->ny < 2 * P->polar_nc + 2))
+    return set_error(KORAL_E_ARG, "grid_create: polar-axis correction needs NY >= 2 NCCORRECTPOLAR + 2");
   for (int k = 0; k < 3; k++)
     if (!D->xb[k]) return set_error(KORAL_E_ARG, "grid_create: missing face coordinates");
   if (!D->gc || !D->Gc || !D->kr || !D->gfx || !D->Gfx || !D->gfy || !D->Gfy)
@@ -497,7 +499,7 @@ const char *koral_hip_grid_op_name(int kind)
 {
   static const char *names[K_NKINDS] = {"p2u", "bc_faces", "bc_edge", "bc_total", "wavespeeds", "flux",
                                         "emf", "ct_faces", "update", "u2p", "fixup", "fixup_apply",
-                                        "implicit", "comb"};
+                                        "implicit", "comb", "polar"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "";
 }
 
@@ -548,6 +550,14 @@ int koral_hip_calc_u2p(koral_hip_grid *g)
   HipExec ex = g->ex();
   u2p(ex, g->b);
   return check_launch("calc_u2p");
+}
+
+int koral_hip_do_correct(koral_hip_grid *g)
+{
+  if (!g) return set_error(KORAL_E_ARG, "null grid");
+  HipExec ex = g->ex();
+  do_correct(ex, g->b);
+  return check_launch("do_correct");
 }
 
 int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt)

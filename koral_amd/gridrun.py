@@ -41,6 +41,7 @@ def _sig(L):
         "koral_hip_tstepden": [_vp, _vp],
         "koral_hip_op_explicit": [_vp, d, d, _vp],
         "koral_hip_calc_u2p": [_vp],
+        "koral_hip_do_correct": [_vp],
         "koral_hip_op_implicit": [_vp, d, d],
         "koral_hip_step_rk2imex": [_vp, d, d, _vp, _vp],
     }.items():
@@ -166,6 +167,9 @@ class DeviceGrid:
 
     def u2p(self):
         check(self.L.koral_hip_calc_u2p(self.h), "calc_u2p")
+
+    def do_correct(self):
+        check(self.L.koral_hip_do_correct(self.h), "do_correct")
 
     def implicit(self, t, dt):
         check(self.L.koral_hip_op_implicit(self.h, t, dt), "op_implicit")

@@ -42,7 +42,7 @@ def _sig(L):
         "koral_hip_mlp_train_step": [vp, vp, vp, i, f, vp, vp],
         "koral_hip_mlp_infer": [vp, vp, l, vp, vp],
         "koral_hip_mlp_guess": [vp, l, vp, vp, vp, vp],
-        "koral_hip_implicit_batch_learned": [vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp],
+        "koral_hip_implicit_batch_learned": [vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp, vp],
     }.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -210,13 +210,15 @@ def standardise(F, T, norm):
     return Fs, Ts
 
 
-def implicit_learned(P, net, uu, pp, g20, G20, dt, global_time=0.0):
-    """koral_hip_implicit_batch_learned in place on uu/pp; (flags, info)."""
+def implicit_learned(P, net, uu, pp, g20, G20, dt, global_time=0.0, maxiter=0):
+    """koral_hip_implicit_batch_learned in place on uu/pp; (flags, info).
+    maxiter > 0 caps the learned attempt's iterations before the plain chain."""
     n = pp.shape[1]
     flags = torch.zeros((7, n), dtype=torch.int32, device=pp.device)
     info = torch.zeros((6, n), dtype=torch.int32, device=pp.device)
     check(net.L.koral_hip_implicit_batch_learned(ctypes.byref(P), net.h, n, _ptr(_chk(uu)), _ptr(_chk(pp)),
                                                  _ptr(_chk(g20)), _ptr(_chk(G20)), _ptr(_chk(dt)),
-                                                 float(global_time), _ptr(flags), _ptr(info), _stream()),
+                                                 float(global_time), int(maxiter), _ptr(flags), _ptr(info),
+                                                 _stream()),
           "implicit_learned")
     return flags, info

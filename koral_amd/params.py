@@ -23,7 +23,7 @@ class KoralParams(ctypes.Structure):
         ("k_boltz", ctypes.c_double), ("m_proton", ctypes.c_double),
         ("sigma_rad", ctypes.c_double), ("mu_gas", ctypes.c_double),
         ("ggg", ctypes.c_double), ("ccc", ctypes.c_double), ("masscm", ctypes.c_double),
-        ("kappa_model", ctypes.c_int), ("pad1", ctypes.c_int),
+        ("kappa_model", ctypes.c_int), ("polar_nc", ctypes.c_int),
         ("kappa_a_coef", ctypes.c_double), ("kappa_es_coef", ctypes.c_double),
         ("rhofloor", ctypes.c_double), ("uurhoratiomin", ctypes.c_double),
         ("uurhoratiomax", ctypes.c_double), ("b2rhoratiomax", ctypes.c_double),

@@ -176,6 +176,11 @@ int oracle_grid_implicit(const oracle_params *P, const o_grid *G, double t, doub
   return o_op_implicit(P, G, t, dt, u, p, flags, counts);
 }
 
+int oracle_grid_do_correct(const oracle_params *P, const o_grid *G, double *u, double *p)
+{
+  return o_do_correct(P, G, u, p);
+}
+
 int oracle_grid_step(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                      double *tstepdenmax, int *counts)
 {

@@ -945,6 +945,66 @@ int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, 
   return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * do_correct (finite.c:605-636) with CORRECT_POLARAXIS: correct_polaraxis
+ * (finite.c:4924-5060) for spherical-like coordinates.  In every (ix, iz)
+ * column the nc cells next to each polar face take the gas and radiation
+ * densities and the r/phi velocities of cell nc (NY-1-nc at the lower face),
+ * the theta velocities scaled by |x2 - x2_axis| / |x2_src - x2_axis| (code
+ * coordinates of cell centres, set_grid finite.c:1630-1660), then u = p2u(p)
+ * for every variable except the field components.  The upper face is
+ * processed before the lower one in each column, as the reference loops.
+ * ------------------------------------------------------------------------- */
+static double x_centre(const o_grid *G, int i, int d)
+{
+  const double *xb = G->xb[d];
+  return .5 * (xb[i + 1 + G->ng] + xb[i + G->ng]);
+}
+
+int o_do_correct(const oracle_params *P, const o_grid *G, double *u, double *p)
+{
+  const int nc = P->polar_nc, nv = P->nv;
+  if (nc <= 0 || G->ny < 2) return 0;
+  const int ee = 6 + 3 * (P->mag ? 1 : 0);
+  double pp[O_NVMAX], uu[O_NVMAX];
+  for (int ix = 0; ix < G->nx; ix++)
+    for (int iz = 0; iz < G->nz; iz++)
+      for (int side = 0; side < 2; side++) {
+        const double thaxis = side == 0 ? G->xb[1][0 + G->ng] : G->xb[1][G->ny + G->ng];
+        const int iysrc = side == 0 ? nc : G->ny - 1 - nc;
+        for (int ic = 0; ic < nc; ic++) {
+          const int iy = side == 0 ? ic : G->ny - 1 - ic;
+          const double th = x_centre(G, iy, 1), thsrc = x_centre(G, iysrc, 1);
+          const double fac = fabs((th - thaxis) / (thsrc - thaxis));
+          const double *ps = p + (long)nv * cidx(G, ix, iysrc, iz);
+          double *pd = p + (long)nv * cidx(G, ix, iy, iz);
+          double *ud = u + (long)nv * cidx(G, ix, iy, iz);
+          for (int iv = 0; iv < nv; iv++) pp[iv] = pd[iv];
+          pp[0] = ps[0];
+          pp[1] = ps[1];
+          pp[5] = ps[5];
+          pp[2] = ps[2];
+          pp[4] = ps[4];
+          pp[3] = fac * ps[3];
+          if (P->rad) {
+            pp[ee] = ps[ee];
+            pp[ee + 1] = ps[ee + 1];
+            pp[ee + 3] = ps[ee + 3];
+            pp[ee + 2] = fac * ps[ee + 2];
+          }
+          o_geom g;
+          o_geom_cell(G, ix, iy, &g);
+          o_p2u(P, pp, uu, &g);
+          for (int iv = 0; iv < nv; iv++) {
+            if (P->mag && iv >= 6 && iv <= 8) continue;
+            pd[iv] = pp[iv];
+            ud[iv] = uu[iv];
+          }
+        }
+      }
+  return 0;
+}
+
 /* RK2IMEX step body: ko.c:444-570 */
 int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                    double *tstepdenmax, int *counts)
@@ -963,20 +1023,24 @@ int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt,
   DOM for (int iv = 0; iv < nv; iv++) C(drt1) = (1. / (dt * gamma)) * C(u) + (-1. / (dt * gamma)) * C(ut0);
   DOM for (int iv = 0; iv < nv; iv++) C(ut1) = 1. * C(u);
   o_calc_u2p(P, G, u, p, flags, counts ? counts + 3 : NULL);
+  o_do_correct(P, G, u, p);
   o_op_explicit(P, G, t, dt, u, p, &tmax);
   DOM for (int iv = 0; iv < nv; iv++) C(dut1) = (1. / (dt)) * C(u) + (-1. / (dt)) * C(ut1);
   DOM for (int iv = 0; iv < nv; iv++) C(u) = C(ut0) + (dt) * C(dut1) + (dt * (1. - 2. * gamma)) * C(drt1);
   DOM for (int iv = 0; iv < nv; iv++) C(uforget) = 1. * C(u);
   o_calc_u2p(P, G, u, p, flags, counts ? counts + 3 : NULL);
+  o_do_correct(P, G, u, p);
   o_op_implicit(P, G, t, gamma * dt, u, p, flags, counts);
   DOM for (int iv = 0; iv < nv; iv++) C(drt2) = (1. / (dt * gamma)) * C(u) + (-1. / (dt * gamma)) * C(uforget);
   DOM for (int iv = 0; iv < nv; iv++) C(ut2) = 1. * C(u);
   o_calc_u2p(P, G, u, p, flags, counts ? counts + 3 : NULL);
+  o_do_correct(P, G, u, p);
   o_op_explicit(P, G, t, dt, u, p, &tmax);
   DOM for (int iv = 0; iv < nv; iv++) C(dut2) = (1. / (dt)) * C(u) + (-1. / (dt)) * C(ut2);
   DOM for (int iv = 0; iv < nv; iv++) C(u) = C(ut0) + (dt / 2.) * C(dut1) + (dt / 2.) * C(dut2);
   DOM for (int iv = 0; iv < nv; iv++) C(u) = C(u) + (dt / 2.) * C(drt1) + (dt / 2.) * C(drt2);
   o_calc_u2p(P, G, u, p, flags, counts ? counts + 3 : NULL);
+  o_do_correct(P, G, u, p);
 #undef DOM
 #undef C
   if (tstepdenmax) *tstepdenmax = tmax;

@@ -31,4 +31,5 @@ int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, 
 int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                    double *tstepdenmax, int *counts);
 double o_tstepden(const oracle_params *P, const o_grid *G, const double *p);
+int o_do_correct(const oracle_params *P, const o_grid *G, double *u, double *p);
 #endif

@@ -76,7 +76,10 @@ typedef struct {
   /* opacities: model 0 -> kappa_abs = kappa_a_coef*rho (RADTUBE kappa.c);
      model 1 -> Kramers free-free in cgs (KATOTORUS kappa.c);
      kappa_es = kappa_es_coef*rho */
-  int kappa_model, pad1;
+  int kappa_model;
+  /* CORRECT_POLARAXIS: NCCORRECTPOLAR cells next to each polar face are
+     overwritten by do_correct (finite.c:605-636); 0 = not defined */
+  int polar_nc;
   double kappa_a_coef, kappa_es_coef;
   /* floors (choices.h:52-118,319-337) */
   double rhofloor, uurhoratiomin, uurhoratiomax, b2rhoratiomax;

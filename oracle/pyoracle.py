@@ -193,6 +193,7 @@ def _grid_lib():
         L.oracle_grid_u2p.argtypes = [vp, vp, _dp, _dp, _ip, _ip]
         L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
         L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
+        L.oracle_grid_do_correct.argtypes = [vp, vp, _dp, _dp]
         L._grid_sigs = True
     return L
 
@@ -246,6 +247,10 @@ class GridState:
         _grid_lib().oracle_grid_u2p(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p),
                                     _i(self.flags), _i(c))
         return c
+
+    def do_correct(self):
+        """do_correct: the polar-axis correction when P.polar_nc > 0."""
+        _grid_lib().oracle_grid_do_correct(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p))
 
     def implicit(self, t, dt):
         c = np.zeros(3, np.int32)

@@ -56,8 +56,10 @@ def _setup(kind):
     if kind == "tube":
         P = KP.params_tube()
         spec, p = PR.tube_setup(P, nx=64)
-    elif kind == "torus_mhd":
+    elif kind in ("torus_mhd", "torus_mhd_polar"):
         P = KP.params_torus_mhd()
+        if kind == "torus_mhd_polar":
+            P.polar_nc = 2
         spec, p = PR.torus_setup(P, n=(16, 16, 8))
     else:
         P = KP.params_torus_rad()
@@ -136,7 +138,7 @@ def test_grid_explicit_parity(oracle, kind):
     assert abs(tm_o - tm_d) / tm_o < 1e-12
 
 
-@pytest.mark.parametrize("kind,nsteps", [("tube", 5), ("torus_mhd", 2),
+@pytest.mark.parametrize("kind,nsteps", [("tube", 5), ("torus_mhd", 2), ("torus_mhd_polar", 2),
                                          pytest.param("torus_rad", 2, marks=RAD_TORUS_OFF)])
 def test_grid_step_parity(oracle, kind, nsteps):
     from koral_amd.gridrun import DeviceGrid
@@ -169,6 +171,26 @@ def test_grid_step_parity(oracle, kind, nsteps):
         t += dt
         dt = 1. / tm_o
         dg.dt = dt
+
+
+@pytest.mark.parametrize("kind", ["torus_mhd", "torus_rad"])
+def test_grid_polar_axis(oracle, kind):
+    """do_correct with the polar-axis correction (NCCORRECTPOLAR = 2)."""
+    from koral_amd.gridrun import DeviceGrid
+    P, spec, p = _setup(kind)
+    P.polar_nc = 2
+    _say("polar-axis correction parity", kind)
+    G, S = _oracle_state(oracle, P, spec, p)
+    S.p[..., 3] += 0.01   # non-zero theta velocities, so the scaling matters
+    if P.rad:
+        S.p[..., P.ee + 2] -= 0.02
+    dg = DeviceGrid(P, spec)
+    dg.upload(S.p, S.u)
+    S.do_correct()
+    dg.do_correct()
+    pd, ud, fd = dg.download()
+    assert np.array_equal(pd, S.p)
+    assert np.array_equal(ud, S.u)
 
 
 def test_field_radiation_explicit_refused():

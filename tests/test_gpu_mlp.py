@@ -171,9 +171,15 @@ def test_learned_guess_solve(dev, oracle):
     _say("same system from both starts (%d cells): |du|/u median %.2e p99 %.2e max %.2e; "
          "velocity max %.2e" % (both.sum(), np.median(du[both]), np.quantile(du[both], 0.99), du[both].max(),
                                 dv[both].max()))
+    for budget in (24, 16, 10):
+        Uc, Pc = U0.clone(), Pp.clone()
+        _, ic = LN.implicit_learned(P, net, Uc, Pc, g, G, dtd, maxiter=budget)
+        ic = ic.cpu().numpy()
+        _say("learned attempt budget %d: solved %d (%d learned), mean iterations %.3f" %
+             (budget, (ic[1] >= 0).sum(), (ic[1] == 6).sum(), ic[5].mean()))
+        assert np.all(ic[1][ok_a] >= 0)
     # cells the learned attempt did not solve follow the plain chain exactly
     assert np.array_equal(pa[same], pb[same])
-    assert np.array_equal(pa[:, 0], pb[:, 0])  # the implicit step keeps rho
     assert learned.sum() > 0.5 * len(ok_a)
     # two converged Newton solves of one system agree to the solver tolerance
     assert np.quantile(du[both], 0.99) < 1e-4 and np.quantile(dv[both], 0.99) < 1e-4

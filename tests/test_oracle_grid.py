@@ -166,3 +166,46 @@ def test_ppm_reproduces_linear_at_faces(oracle):
     xb = spec.xb[0][spec.ng + 3: spec.ng + n - 2]
     rel = np.abs(rho_face[spec.ngx + 3: spec.ngx + n - 2] - lin(xb)) / lin(xb)
     assert rel.max() < 1e-13
+
+
+@pytest.mark.parametrize("rad", [0, 1])
+def test_polar_axis_correction(oracle, rad):
+    """do_correct with CORRECT_POLARAXIS (finite.c:4924-5060) on a torus in
+    MKS1 coordinates, whose x2 = theta faces are uniform: the nc cells next to
+    each polar face take rho, u, S, v^r, v^phi (and E, F^r, F^phi) of cell nc
+    (NY-1-nc), their theta components scaled by (ic + 1/2) / (nc + 1/2); the
+    field is left alone; u = p2u(p) there; every other cell is untouched."""
+    P = KP.params_torus_rad() if rad else KP.params_torus_mhd()
+    P.polar_nc = 2
+    spec, p = PR.torus_setup(P, n=(8, 12, 4))
+    S = oracle.GridState(P, oracle.Grid(spec), GR.to_aos(p))
+    S.init_u()
+    S.set_bc()
+    # make the theta components non-trivial so the scaling is visible
+    S.p[..., 3] += 0.01
+    if rad:
+        S.p[..., P.ee + 2] -= 0.02
+    p0, u0 = S.p.copy(), S.u.copy()
+    S.do_correct()
+    nc, ny, ngy = P.polar_nc, spec.ny, spec.ngy
+    zs, xs = spec.interior()[0], spec.interior()[2]
+    gas = [0, 1, 5, 2, 4] + ([P.ee, P.ee + 1, P.ee + 3] if rad else [])
+    theta = [3] + ([P.ee + 2] if rad else [])
+    touched = np.zeros(S.p.shape[:-1], bool)
+    for side in (0, 1):
+        src = ngy + (nc if side == 0 else ny - 1 - nc)
+        for ic in range(nc):
+            iy = ngy + (ic if side == 0 else ny - 1 - ic)
+            touched[zs, iy, xs] = True
+            a, b = S.p[zs, iy, xs], p0[zs, src, xs]
+            assert np.array_equal(a[..., gas], b[..., gas])
+            fac = (ic + 0.5) / (nc + 0.5)
+            np.testing.assert_allclose(a[..., theta], fac * b[..., theta], rtol=1e-12, atol=0)
+            assert np.array_equal(a[..., 6:9], p0[zs, iy, xs][..., 6:9])  # field kept
+            assert np.array_equal(S.u[zs, iy, xs][..., 6:9], u0[zs, iy, xs][..., 6:9])
+    # u = p2u(p) on the corrected cells (independent per-point p2u)
+    gc, Gc = spec.cell_tables()
+    uu = oracle.p2u(P, S.p[touched], np.ascontiguousarray(gc[touched]), np.ascontiguousarray(Gc[touched]))
+    keep = [iv for iv in range(P.nv) if not 6 <= iv <= 8]
+    assert np.array_equal(S.u[touched][:, keep], uu[:, keep])
+    assert np.array_equal(S.p[~touched], p0[~touched]) and np.array_equal(S.u[~touched], u0[~touched])
