@@ -185,4 +185,5 @@ def params_torus_rad(mass_msun=10.0):
     P.int_order = 4
     P.fluxmethod = 1
     P.tsteplim = 0.5
+    P.polar_nc = 2  # CORRECT_POLARAXIS, NCCORRECTPOLAR 2 (PROBLEMS/KATOTORUS/define.h:234-238)
     return P

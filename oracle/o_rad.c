@@ -383,9 +383,13 @@ int oracle_inverse44(const double *a, double *ia)
 }
 
 /* solve_implicit_lab_4dprim: rad.c:376-1218 -- damped Newton on 4 primitives
-   with a one-sided finite-difference Jacobian */
-int o_solve_implicit_4dprim(const oracle_params *P, const double *uu00, const double *pp00, const o_geom *g,
-                            double dt, const int *params_in, double *pp, double global_time, int *iters_out)
+   with a one-sided finite-difference Jacobian.  x0 (may be NULL) replaces the
+   start of the four iterated primitives (the reference starts from the old
+   state); maxiter > 0 lowers RADIMPMAXITER for this attempt.  Both serve the
+   learned initial guess, which has no reference counterpart. */
+int o_solve_implicit_4dprim_from(const oracle_params *P, const double *uu00, const double *pp00, const o_geom *g,
+                                 double dt, const int *params_in, double *pp, double global_time, int *iters_out,
+                                 const double *x0, int maxiter)
 {
   int nv = P->nv, ee = ee_of(P);
   int params[4] = {params_in[0], params_in[1], params_in[2], params_in[3]};
@@ -402,7 +406,10 @@ int o_solve_implicit_4dprim(const oracle_params *P, const double *uu00, const do
   double EPS = P->radimpeps;
   double CONV = (params[1] == O_ENTROPYEQ) ? P->radimpentrconv : P->radimpconv;
   int MAXITER = P->radimpmaxiter;
+  if (maxiter > 0 && maxiter < MAXITER) MAXITER = maxiter;
   int sh = (whichprim == O_MHD) ? 1 : ee;
+  if (x0)
+    for (int i = 0; i < 4; i++) pp[i + sh] = x0[i];
   int iter = 0, failed = 0;
   *iters_out = 0;
   for (;;) {
@@ -546,10 +553,28 @@ int o_solve_implicit_4dprim(const oracle_params *P, const double *uu00, const do
   return 0;
 }
 
+int o_solve_implicit_4dprim(const oracle_params *P, const double *uu00, const double *pp00, const o_geom *g,
+                            double dt, const int *params_in, double *pp, double global_time, int *iters_out)
+{
+  return o_solve_implicit_4dprim_from(P, uu00, pp00, g, dt, params_in, pp, global_time, iters_out, NULL, 0);
+}
+
 /* solve_implicit_lab: rad.c:1225-1570, with implicit_lab_rad_source_term
    rad.c:2758-2795 flag handling.  uu/pp are updated in place on success. */
 int o_solve_implicit_lab(const oracle_params *P, double *uu, double *pp, const o_geom *g, double dt,
                          double global_time, int *flags, o_imp_info *info)
+{
+  return o_solve_implicit_lab_guess(P, uu, pp, g, dt, global_time, flags, info, NULL, 0);
+}
+
+/* solve_implicit_lab preceded, when gx is given, by one attempt (energy
+   equation, lab frame, gas primitives) from the start (u, u~^1..3) = gx with
+   at most maxiter iterations; a cell it solves reports attempt 6, otherwise
+   the unchanged six-attempt chain runs from the old state and total_iters
+   includes the extra attempt.  The learned initial guess has no reference
+   counterpart. */
+int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, const o_geom *g, double dt,
+                               double global_time, int *flags, o_imp_info *info, const double *gx, int maxiter)
 {
   int nv = P->nv;
   double pp00[O_NVMAX], uu00[O_NVMAX], ppn[O_NVMAX];
@@ -569,9 +594,17 @@ int o_solve_implicit_lab(const oracle_params *P, double *uu, double *pp, const o
   /* fallback chain, rad.c:1389-1464 */
   const int seq[6][3] = {{0, O_ENERGYEQ, O_LAB}, {0, O_ENERGYEQ, O_FF}, {1, O_ENERGYEQ, O_LAB},
                          {1, O_ENERGYEQ, O_FF},  {0, O_ENTROPYEQ, O_LAB}, {1, O_ENTROPYEQ, O_LAB}};
-  int ret = -1, total = 0, it = 0, a;
+  int ret = -1, total = 0, it = 0, a = 0;
   int params[4] = {0, 0, 0, 0};
-  for (a = 0; a < 6; a++) {
+  if (gx) {
+    params[0] = O_MHD;
+    params[1] = O_ENERGYEQ;
+    params[2] = O_LAB;
+    ret = o_solve_implicit_4dprim_from(P, uu00, pp00, g, dt, params, ppn, global_time, &it, gx, maxiter);
+    total += it;
+    if (ret == 0) a = 6;
+  }
+  for (; ret != 0 && a < 6; a++) {
     params[0] = seq[a][0] ? other : startwith;
     params[1] = seq[a][1];
     params[2] = seq[a][2];

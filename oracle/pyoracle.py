@@ -40,6 +40,8 @@ def lib():
         L.oracle_ff_Rtt_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp]
         L.oracle_implicit_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                             _ip, _ip, ctypes.c_int]
+        L.oracle_implicit_batch_guess.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                                  _ip, _ip, ctypes.c_int, _dp, ctypes.c_int]
         L.oracle_inverse44.argtypes = [_dp, _dp]
         L.oracle_inverse44.restype = ctypes.c_int
         _lib = L
@@ -128,19 +130,27 @@ def ff_Rtt(P, pp, g20, G20):
     return out
 
 
-def implicit(P, uu, pp, g20, G20, dt, global_time=0.0, nthreads=1):
+def implicit(P, uu, pp, g20, G20, dt, global_time=0.0, nthreads=1, gx=None, maxiter=0):
     """implicit_lab_rad_source_term on independent cells.
     Returns (uu, pp, flags[n,7], info[n,6]) with info columns
-    iters, attempt, whichprim, whicheq, whichframe, total_iters."""
+    iters, attempt, whichprim, whicheq, whichframe, total_iters.
+    gx [n, 4] (u, u~^1..3), when given, is the start of a first attempt on
+    the gas primitives (energy equation, lab frame, at most maxiter
+    iterations when maxiter > 0), reported as attempt 6 when it succeeds."""
     uu = np.array(uu, dtype=np.float64, order="C")
     pp = np.array(pp, dtype=np.float64, order="C")
     n = pp.shape[0]
     flags = np.zeros((n, 7), np.int32)
     info = np.zeros((n, 6), np.int32)
     dt = np.ascontiguousarray(dt, dtype=np.float64)
-    lib().oracle_implicit_batch(_P(P), n, _d(uu), _d(pp), _d(np.ascontiguousarray(g20)),
-                                _d(np.ascontiguousarray(G20)), _d(dt), global_time, _i(flags), _i(info),
-                                nthreads)
+    gxp = None
+    if gx is not None:
+        gx = np.ascontiguousarray(gx, dtype=np.float64)
+        assert gx.shape == (n, 4)
+        gxp = _d(gx)
+    lib().oracle_implicit_batch_guess(_P(P), n, _d(uu), _d(pp), _d(np.ascontiguousarray(g20)),
+                                      _d(np.ascontiguousarray(G20)), _d(dt), global_time, _i(flags), _i(info),
+                                      nthreads, gxp, int(maxiter))
     return uu, pp, flags, info
 
 

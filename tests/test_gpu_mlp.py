@@ -184,3 +184,45 @@ def test_learned_guess_solve(dev, oracle):
     # two converged Newton solves of one system agree to the solver tolerance
     assert np.quantile(du[both], 0.99) < 1e-4 and np.quantile(dv[both], 0.99) < 1e-4
     assert it_b < it_a
+
+
+def test_learned_solve_matches_oracle(dev, oracle):
+    """The device solve with the learned first attempt against the oracle's
+    solve from the same start (downloaded from the network kernels): same
+    attempt indices and iteration counts, primitives to the parity bar of the
+    plain implicit solve."""
+    import torch
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd import learn as LN
+    P = KP.params_implicit_batch()
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 16, seed=41)
+    Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    U = cell.p2u(P, Pp, g, G)
+    P0 = Pp.clone()
+    _, info = cell.implicit(P, U, Pp, g, G, dtd)
+    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
+    norm = LN.normalisation(F, T)
+    Fs, Ts = LN.standardise(F, T, norm)
+    net = LN.Net(seed=1)
+    net.set(LN.init_params(1), norm)
+    net.fit(Fs, Ts, epochs=3, lr=1e-3)
+    n = 3000
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=42)
+    uu = oracle.p2u(P, pp, g20, G20)
+    Pd, Ud, gd, Gd = cell.to_dev(pp), cell.to_dev(uu), cell.to_dev(g20), cell.to_dev(G20)
+    dtd = torch.from_numpy(dt).to(dev)
+    gx = net.guess(Pd, net.infer(net.features(P, Pd, dtd))).cpu().numpy().T.copy()
+    for budget in (0, 12):
+        U1, P1 = Ud.clone(), Pd.clone()
+        fl, info = LN.implicit_learned(P, net, U1, P1, gd, Gd, dtd, maxiter=budget)
+        info = info.cpu().numpy().T
+        po = cell.to_host(P1)
+        uo_r, po_r, fl_r, info_r = oracle.implicit(P, uu, pp, g20, G20, dt, gx=gx, maxiter=budget)
+        same = np.all(info == info_r, axis=1)
+        ok = same & (info[:, 1] >= 0)
+        rel = np.abs(po[ok] - po_r[ok]) / (np.abs(po[ok]) + np.abs(po_r[ok]) + 1e-300)
+        _say("budget %d: %d cells, learned attempt %d, info identical %.4f, max rel p %.2e" %
+             (budget, n, (info[:, 1] == 6).sum(), same.mean(), rel.max()))
+        assert (info[:, 1] == 6).sum() > 0
+        assert same.mean() > 0.99
+        assert rel.max() < 1e-9

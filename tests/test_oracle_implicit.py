@@ -118,3 +118,38 @@ def test_zero_opacity_is_identity(oracle):
     scale = np.abs(uu[:, 1:5]).max(1) + np.abs(uu[:, P.ee:P.ee + 4]).max(1)
     keep = [i for i in range(P.nv) if i != 5]
     assert (np.abs(uo - uu)[:, keep].max(1) / scale < 1e-13).all()
+
+
+def test_guess_start_fixed_points(oracle):
+    """The extra first attempt of the learned initial guess (no reference
+    counterpart) is pinned by two starts whose outcome is known:
+    - the old state itself: the attempt is the reference's attempt 0 on gas
+      primitives, so a cell the plain chain solves there is solved with the
+      same iterations and the same state, and a cell it fails there runs the
+      plain chain after an identical failed attempt;
+    - the converged solution: the residual is already below tolerance, so the
+      attempt stops after one iteration on the same state."""
+    P, pp, g20, G20, dt = _batch(800, seed=33)
+    uu = oracle.p2u(P, pp, g20, G20)
+    ua, pa, fa, ia = oracle.implicit(P, uu, pp, g20, G20, dt)
+    gas0 = (ia[:, 1] == 0) & (ia[:, 2] == 2)          # attempt 0 on gas primitives
+    start_gas = np.isin(ia[:, 1], [0, 1, 4]) & (ia[:, 2] == 2) | np.isin(ia[:, 1], [2, 3, 5]) & (ia[:, 2] == 1)
+    assert gas0.sum() > 100
+    # start = old state
+    ub, pb, fb, ib = oracle.implicit(P, uu, pp, g20, G20, dt, gx=pp[:, 1:5])
+    assert np.all(ib[gas0, 1] == 6)
+    assert np.array_equal(ib[gas0, 0], ia[gas0, 0]) and np.array_equal(ib[gas0, 5], ia[gas0, 5])
+    assert np.array_equal(pb[gas0], pa[gas0]) and np.array_equal(ub[gas0], ua[gas0])
+    # gas-start cells that the plain chain solved later: one failed extra
+    # attempt identical to the plain attempt 0, then the same chain
+    later = start_gas & (ia[:, 1] > 0)
+    assert later.sum() > 0
+    assert np.array_equal(ib[later, 1], ia[later, 1]) and np.array_equal(pb[later], pa[later])
+    assert np.all(ib[later, 5] > ia[later, 5])
+    # start = converged solution of the same system
+    uc, pc, fc, ic = oracle.implicit(P, uu, pp, g20, G20, dt, gx=pa[:, 1:5])
+    assert np.all(ic[gas0, 1] == 6) and np.all(ic[gas0, 0] == 1)
+    assert np.array_equal(pc[gas0], pa[gas0])
+    # a budget of one iteration still solves them, and fails the others
+    ud, pd, fd, idd = oracle.implicit(P, uu, pp, g20, G20, dt, gx=pa[:, 1:5], maxiter=1)
+    assert np.all(idd[gas0, 1] == 6)
