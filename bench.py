@@ -157,24 +157,22 @@ def bench_learned(args):
     U = cell.p2u(P, Pp, g, G)
     P0 = Pp.clone()
     _, info = cell.implicit(P, U, Pp, g, G, dtd)
-    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
+    F, Tg, Tr = LN.training_pairs(P, P0, Pp, dtd, info)
     del U, Pp, P0, g, G, dtd, info
     torch.cuda.empty_cache()
-    norm = LN.normalisation(F, T)
-    Fs, Ts = LN.standardise(F, T, norm)
-    del F, T
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
-    say("%d training pairs from %d cells in %.1f s" % (Fs.shape[1], args.train_cells, t_pairs))
-    net = LN.Net(seed=0)
-    net.set(LN.init_params(0), norm)
+    n_pairs = F.shape[1]
+    say("%d training pairs from %d cells in %.1f s" % (n_pairs, args.train_cells, t_pairs))
+    sched = [(1e-3, args.epochs), (3e-4, max(1, args.epochs // 2)), (1e-4, max(1, args.epochs // 4))]
     t0 = time.perf_counter()
-    hist = net.fit(Fs, Ts, epochs=args.epochs, lr=1e-3)
+    gas, hist_g = LN.train(P, F, Tg, LN.GAS, schedule=sched)
+    rad, hist_r = LN.train(P, F, Tr, LN.RAD, schedule=sched)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    say("trained %d epochs in %.1f s, loss %s" % (args.epochs, t_train, hist))
-    n_pairs = Fs.shape[1]
-    del Fs, Ts
+    say("trained both networks (%s) in %.1f s, final loss gas %.4f rad %.4f" %
+        (sched, t_train, hist_g[-1], hist_r[-1]))
+    del F, Tg, Tr
     torch.cuda.empty_cache()
     # ---- fresh evaluation batch ----
     n = args.cells
@@ -191,7 +189,7 @@ def bench_learned(args):
                                               args.steps, args.warmup)
     pa = Pp.clone()
     say("plain solve %.2f ms" % (w_plain * 1e3))
-    w_learn, k_learn, (_, ib) = _timed_solves(lambda: LN.implicit_learned(P, net, U, Pp, g, G, dtd,
+    w_learn, k_learn, (_, ib) = _timed_solves(lambda: LN.implicit_learned(P, gas, rad, U, Pp, g, G, dtd,
                                                                           maxiter=args.learned_maxiter), reset,
                                               args.steps, args.warmup)
     pb = Pp.clone()
@@ -200,25 +198,25 @@ def bench_learned(args):
     stream = torch.cuda.current_stream()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     ev[0].record(stream)
-    Fe = net.features(P, Pp, dtd)
+    Fe = gas.features(P, Pp, dtd)
     ev[1].record(stream)
-    Ye = net.infer(Fe)
+    Ye = gas.infer(Fe)
     ev[2].record(stream)
-    net.guess(Pp, Ye)
+    gas.guess(Pp, Ye)
     ev[3].record(stream)
     ev[3].synchronize()
     t_feat, t_inf, t_guess = (ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3))
     ia, ib = ia.cpu().numpy(), ib.cpu().numpy()
     ok_a, ok_b = ia[1] >= 0, ib[1] >= 0
     pa, pb = pa.cpu().numpy(), pb.cpu().numpy()
-    both = (ib[1] == 6) & (ia[1] == 0) & (ia[2] == 2)
+    both = (ib[1] == 6) & (ia[1] == 0) & (ia[2] == 2) & (ib[2] == 2)
     du = np.abs(pa[1] - pb[1]) / np.abs(pa[1])
     it_a, it_b = float(ia[5].mean()), float(ib[5].mean())
     # algorithmic flops: the counted FP64 operations of the plain solve scaled
     # per Newton iteration to the learned solve's iterations, plus the network
     flops_it = IMPLICIT_FLOPS_PER_SOLVE / it_a
-    mlp_flops = 2.0 * (16 * 128 + 2 * 128 * 128 + 128 * 16)   # padded MFMA tiles
-    learned_flops = flops_it * it_b + mlp_flops
+    mlp_flops = 2.0 * (16 * 128 + 2 * 128 * 128 + 128 * 16)   # padded MFMA tiles, per network
+    learned_flops = flops_it * it_b + 2 * mlp_flops
     achieved = learned_flops * n / k_learn / 1e12
     inf_tf = mlp_flops * n / t_inf / 1e12
     res = {
@@ -236,15 +234,16 @@ def bench_learned(args):
         "data": "synthetic (seeded)",
         "config": {"workload": "learned-guess implicit radiation-source solve vs plain guess",
                    "cells": n, "train_cells": args.train_cells, "train_pairs": int(n_pairs),
-                   "epochs": args.epochs, "batch": LN.BATCH, "lr": 1e-3,
+                   "schedule_lr_epochs": sched, "batch": LN.BATCH, "networks": "gas + radiation",
                    "learned_maxiter": args.learned_maxiter or P.radimpmaxiter,
-                   "train_loss_per_epoch": hist, "pairs_s": t_pairs, "train_s": t_train,
+                   "train_loss_gas": hist_g, "train_loss_rad": hist_r, "pairs_s": t_pairs, "train_s": t_train,
                    "plain_solves_per_s": n / w_plain, "learned_solves_per_s": n / w_learn,
                    "speedup": w_plain / w_learn,
                    "mean_total_iters_plain": it_a, "mean_total_iters_learned": it_b,
                    "iteration_ratio": it_a / it_b,
                    "solved_frac_plain": float(ok_a.mean()), "solved_frac_learned": float(ok_b.mean()),
                    "learned_attempt_frac": float((ib[1] == 6).mean()),
+                   "learned_rad_prims_frac": float(((ib[1] == 6) & (ib[2] == 1)).mean()),
                    "same_system_cells": int(both.sum()),
                    "same_system_du_median": float(np.median(du[both])) if both.any() else None,
                    "same_system_du_max": float(du[both].max()) if both.any() else None,
@@ -252,8 +251,8 @@ def bench_learned(args):
                                   "guess": t_guess * 1e3}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "learned implicit solve: features + BF16 inference + guess + classify + "
-                               "13 k_imp_pass launches (one batch)",
+                     "kernel": "learned implicit solve: 2 x (features + BF16 inference + guess) + classify + "
+                               "14 k_imp_pass launches (one batch)",
                      "kernel_ms": k_learn * 1e3,
                      "algorithmic_flops_per_launch": learned_flops * n,
                      "flops_basis": "counted FP64 operations per plain solve / plain mean iterations x "

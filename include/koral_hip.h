@@ -105,25 +105,28 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
    counterpart: KORAL starts the Newton solve from the old state) ----
 
    A small network (11 -> 128 -> 128 -> 128 -> 4, GELU) maps the local state
-   of a cell to corrections of the Newton start (u, u~^1..3).  Parameters are
+   of a cell to corrections of the Newton start.  kind 0 (gas) corrects
+   (u, u~^1..3), kind 1 (radiation) corrects (E, F~^1..3).  Parameters are
    float32 [koral_hip_mlp_nparam()] in the order W1[128][11], b1[128],
    W2[128][128], b2, W3[128][128], b3, W4[4][128], b4 (W[out][in]); norm is
    float32 [30]: feature mean[11], std[11], target mean[4], std[4].
    Features (feat[k*n+i], standardised when a network is given): log10 rho,
    log10(u/rho), log10(E/u), gas u~^1..3, radiation u~^1..3,
-   log10(dt kappa_abs), log10(dt kappa_es).  Targets (targ[k*n+i]):
-   log(us/u0), us~^i - u0~^i between converged (pp1, "s") and initial (pp0)
-   primitives.  Training batches are [11][B] / [4][B] with B <= 4096. */
+   log10(dt kappa_abs), log10(dt kappa_es).  Targets (targ[k*n+i]) between
+   converged (pp1) and initial (pp0) primitives x of the kind:
+   log(x1_0/x0_0), asinh((x1_k - x0_k)/1e-6) for k = 1..3.  Training batches
+   are [11][B] / [4][B] with B <= 4096. */
 typedef struct koral_hip_mlp koral_hip_mlp;
 int koral_hip_mlp_nparam(void);
-int koral_hip_mlp_create(koral_hip_mlp **out);
+int koral_hip_mlp_create(int kind, koral_hip_mlp **out);
 int koral_hip_mlp_destroy(koral_hip_mlp *m);
+int koral_hip_mlp_kind(const koral_hip_mlp *m);
 int koral_hip_mlp_set(koral_hip_mlp *m, const float *theta, const float *norm);  /* host arrays; resets Adam */
 int koral_hip_mlp_get(koral_hip_mlp *m, float *theta, float *norm);              /* host arrays, may be NULL */
 int koral_hip_mlp_features(const koral_params *P, const koral_hip_mlp *m, long n, const double *pp,
                            const double *dt, float *feat, void *stream);
-int koral_hip_mlp_targets(const koral_hip_mlp *m, long n, int nv, const double *pp0, const double *pp1,
-                          float *targ, void *stream);
+int koral_hip_mlp_targets(int kind, long n, int nv, const double *pp0, const double *pp1, float *targ,
+                          void *stream);
 /* FP32 forward pass of a training batch, Y [4][B] */
 int koral_hip_mlp_forward(koral_hip_mlp *m, const float *X, int B, float *Y, void *stream);
 /* loss (host, may be NULL; synchronises) and gradient (device [nparam], may be NULL) of the
@@ -135,16 +138,23 @@ int koral_hip_mlp_train_step(koral_hip_mlp *m, const float *X, const float *T, i
                              void *stream);
 /* BF16 matrix-core inference: standardised outputs Y [4][n] */
 int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, void *stream);
-/* Newton start gx [4][n] (u, u~^1..3) from the outputs (the plain start where not finite/positive) */
-int koral_hip_mlp_guess(koral_hip_mlp *m, long n, const double *pp, const float *Y, double *gx, void *stream);
+/* Newton start gx [4][n] of the network's kind from the outputs (the old state where not
+   finite or not positive) */
+int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, const float *Y, double *gx,
+                        void *stream);
 /* koral_hip_implicit_batch with a first attempt from the learned guess (energy equation, lab
-   frame, MHD primitives) under the unchanged convergence tolerance; cells it does not solve
-   within maxiter iterations (RADIMPMAXITER when maxiter <= 0 or larger) run the unchanged
-   six-attempt chain from the plain start.  info[1*n+i] = 6 marks a cell solved by the
-   learned attempt; info[5*n+i] counts the iterations of all attempts. */
-int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, double *uu, double *pp,
-                                     const double *g20, const double *G20, const double *dt, double global_time,
-                                     int maxiter, int *flags, int *info, void *stream);
+   frame) under the unchanged convergence tolerance.  With a radiation network (rad may be
+   NULL) a cell iterates the radiation primitives from its guess when the gas guess,
+   completed by conservation, has E^ < RADIMPLICITTHRESHOLD u or no valid radiation state
+   (the criterion of rad.c:1381 applied to the predicted state; its gas part starts from the
+   gas guess), and the gas primitives from the gas guess otherwise.  Cells the attempt does not solve within maxiter iterations
+   (RADIMPMAXITER when maxiter <= 0 or larger) run the unchanged six-attempt chain from the
+   old state.  info[1*n+i] = 6 marks a cell solved by the learned attempt; info[5*n+i]
+   counts the iterations of all attempts. */
+int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
+                                     double *uu, double *pp, const double *g20, const double *G20,
+                                     const double *dt, double global_time, int maxiter, int *flags, int *info,
+                                     void *stream);
 
 
 /* ---- grid operators (device-resident state, one GPU holding the whole

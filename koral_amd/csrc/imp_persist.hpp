@@ -15,10 +15,15 @@
 // scheme, to the list of the next attempt.  Cells need 1 to 51 iterations
 // per attempt, so no lane waits for the slowest cell of its wave.
 //
-// With a learned initial guess (gx != null) a first pass runs before the
-// chain: the energy-equation lab-frame solve on MHD primitives from the guess
-// (u, u~^1..3 replaced by gx[0..3][c]) over all cells; a cell it does not
-// solve within the iteration budget enters attempt 0 of the plain chain
+// With a learned initial guess a first pass runs before the chain: the
+// energy-equation lab-frame solve from a start given per cell.  The learned
+// classification puts a cell on the radiation primitives, starting from the
+// radiation network's guess, when the gas network's guess completed by
+// conservation has no valid radiation state or E^ < RADIMPLICITTHRESHOLD u
+// (the choice rad.c:1381 makes on the old state, applied to the predicted
+// one), and on the gas primitives from the gas guess otherwise; without a
+// radiation guess every cell takes the gas primitives.  A cell the pass does
+// not solve within its iteration budget enters attempt 0 of the plain chain
 // unchanged, so the guess can change the iteration count but not what the
 // chain would do without it.  A cell solved by that pass reports attempt 6.
 //
@@ -37,9 +42,12 @@ namespace kd {
 // Device buffers of the passes: two list pairs (ping-pong, each n entries per
 // primitive kind) and IMP_NCTR counters.  Counter 4 a + k: k = 0, 1 the list
 // lengths for attempt a (MHD, RAD); k = 2, 3 the work counters of its two
-// kernels; counter IMP_LEARN_Q the work counter of the learned-guess pass.
+// kernels.  The learned pass reads list pair 1 (free until attempt 0 writes
+// its failures there) with lengths IMP_LEARN_LEN + k and work counters
+// IMP_LEARN_Q + k.
 constexpr int IMP_NCTR = 32;
-constexpr int IMP_LEARN_Q = 28;
+constexpr int IMP_LEARN_LEN = 28;
+constexpr int IMP_LEARN_Q = 30;
 constexpr int IMP_LEARNED_ATTEMPT = 6;
 struct ImpBufs {
   unsigned *list[2][2];
@@ -72,42 +80,67 @@ __device__ __forceinline__ void wave_append(bool pred, unsigned v, unsigned *lis
 
 // sorts the n cells into the two attempt-0 lists by the primitives the
 // chain starts with
-// (without lists, for the learned pass, only the per-cell init)
 template <int MAG, int RAD, class Src>
-__global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, long n, ImpBufs b, int lists)
+__global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, long n, ImpBufs b)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int wp = 0;
   if (c < n) {
     src.init(c);
-    if (lists) {
-      double uu00[NV], pp00[NV], dt;
-      Geom g;
-      src.load(c, uu00, pp00, g, dt);
-      wp = imp_startwith<MAG, RAD>(P, pp00, g);
-    }
+    double uu00[NV], pp00[NV], dt;
+    Geom g;
+    src.load(c, uu00, pp00, g, dt);
+    wp = imp_startwith<MAG, RAD>(P, pp00, g);
   }
   wave_append(wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
   wave_append(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
+}
+
+// learned classification: the primitive kind of every cell's learned
+// attempt (see above); gxr may be null
+template <int MAG, class Src>
+__global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Src src, long n, ImpBufs b,
+                                                              const double *__restrict__ gxg,
+                                                              const double *__restrict__ gxr)
+{
+  constexpr int NV = Lay<MAG, 1>::NV;
+  constexpr int EE = Lay<MAG, 1>::EE;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int wp = 0;
+  if (c < n) {
+    src.init(c);
+    wp = P_MHD;
+    if (gxr) {
+      double uu00[NV], pp00[NV], dt, uu[NV];
+      Geom g;
+      src.load(c, uu00, pp00, g, dt);
+#pragma unroll
+      for (int i = 0; i < 4; i++) pp00[1 + i] = gxg[i * n + c];
+      const int r = complete_state<MAG, 1, P_MHD>(P, pp00, uu, uu00, g, false);
+      if (r != 0 || imp_startwith<MAG, 1>(P, pp00, g) == P_RAD) wp = P_RAD;
+    }
+  }
+  wave_append(wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
+  wave_append(wp == P_RAD, (unsigned)c, b.list[1][1], &b.ctr[IMP_LEARN_LEN + 1]);
 }
 
 // attempt a on the cells of one list, all iterating on primitives WP with
 // equation WE in frame WF
 template <int MAG, int RAD, int WP, int WE, int WF, bool LEARN, class Src>
 __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, double global_time,
-                                                  int *counts, const double *gx, long n_all)
+                                                  int *counts, const double *gxg, const double *gxr, long n_all)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const int lane = threadIdx.x & 63;
   const int kind = (WP == P_MHD) ? 0 : 1;
   // the learned pass takes every cell in index order and hands failures to
   // attempt 0; attempt a takes its list and hands failures to attempt a + 1
-  const unsigned *list = LEARN ? nullptr : b.list[a & 1][kind];
+  const unsigned *list = LEARN ? b.list[1][kind] : b.list[a & 1][kind];
   unsigned *out0 = LEARN ? b.list[0][0] : b.list[(a + 1) & 1][0];
   unsigned *out1 = LEARN ? b.list[0][1] : b.list[(a + 1) & 1][1];
-  const long n = LEARN ? n_all : (long)b.ctr[4 * a + kind];
-  unsigned *queue = LEARN ? &b.ctr[IMP_LEARN_Q] : &b.ctr[4 * a + 2 + kind];
+  const long n = LEARN ? (long)b.ctr[IMP_LEARN_LEN + kind] : (long)b.ctr[4 * a + kind];
+  unsigned *queue = LEARN ? &b.ctr[IMP_LEARN_Q + kind] : &b.ctr[4 * a + 2 + kind];
   unsigned *out_len = LEARN ? &b.ctr[0] : &b.ctr[4 * (a + 1)];
   int next_kind = LEARN ? 0 : ((imp_next_wp(a, WP) == P_MHD) ? 0 : 1);
   bool active = false, fresh = false, exhausted = n == 0;
@@ -133,7 +166,7 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
         if (!active) {
           const long k = (long)base + __popcll(need & ((1ull << lane) - 1ull));
           if (k < n) {
-            c = LEARN ? (unsigned)k : list[k];
+            c = list[k];
             active = true;
             fresh = true;
           }
@@ -149,8 +182,14 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
 #pragma unroll
         for (int i = 0; i < NV; i++) pp[i] = pp00[i];
         if (LEARN) {
+          // the gas guess also starts the gas inversion of a radiation-primitive attempt
 #pragma unroll
-          for (int i = 0; i < 4; i++) pp[1 + i] = gx[i * n_all + c];
+          for (int i = 0; i < 4; i++) pp[1 + i] = gxg[i * n_all + c];
+          if (WP == P_RAD) {
+            constexpr int EE = Lay<MAG, RAD>::EE;
+#pragma unroll
+            for (int i = 0; i < 4; i++) pp[EE + i] = gxr[i * n_all + c];
+          }
         }
         iter = 0;
         fresh = false;
@@ -217,30 +256,35 @@ int resident_blocks(K kernel, int bs, long n, unsigned *blocks)
 
 template <int MAG, int WP, int WE, int WF, bool LEARN = false, class Src>
 int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s, const double *gx = nullptr)
+                    hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr)
 {
   unsigned blocks = 0;
   int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src>, 256, n, &blocks);
   if (rc) return rc;
-  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts, gx, n);
+  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts, gxg, gxr, n);
   return 0;
 }
 
-// all passes for n cells (asynchronous on s); counts may be null; gx, when
-// given, holds the learned guess [4][n] of (u, u~^1..3), and the learned
-// pass stops a cell after learned_maxiter iterations when that is positive
-// and below RADIMPMAXITER
+// all passes for n cells (asynchronous on s); counts may be null.  With a
+// learned guess, gxg [4][n] is the gas start (u, u~^1..3) and gxr [4][n] (may
+// be null) the radiation start (E, F~^1..3); the learned pass stops a cell
+// after learned_maxiter iterations when that is positive and below
+// RADIMPMAXITER.
 template <int MAG, class Src>
 int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s, const double *gx = nullptr, int learned_maxiter = 0)
+                    hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr, int learned_maxiter = 0)
 {
+  const unsigned nb = (unsigned)((n + 255) / 256);
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
-  k_imp_classify<MAG, 1, Src><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, src, n, b, gx ? 0 : 1);
   int rc = 0;
-  if (gx) {
+  if (gxg) {
+    k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxg, gxr);
     koral_params Pl = P;
     if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
-    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gx);
+    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gxg, gxr);
+    if (!rc && gxr) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gxg, gxr);
+  } else {
+    k_imp_classify<MAG, 1, Src><<<nb, 256, 0, s>>>(P, src, n, b);
   }
   for (int a = 0; a < 6 && !rc; a++) {
     if (a >= 4) {

@@ -199,9 +199,9 @@ struct ImpSet {
   hipStream_t last = nullptr;
   ImpBufs b{};
   void *mem = nullptr;
-  void *lmem = nullptr;  // learned-guess buffers (features, outputs, guess), made on first use
+  void *lmem = nullptr;  // learned-guess buffers (features, outputs, guesses), made on first use
   float *feat = nullptr, *y = nullptr;
-  double *gx = nullptr;
+  double *gxg = nullptr, *gxr = nullptr;
   hipEvent_t done = nullptr;
 };
 std::mutex pool_mu;
@@ -247,11 +247,12 @@ int learned_buffers(ImpSet *s)
 {
   if (s->lmem) return 0;
   const size_t cap = (size_t)s->cap;
-  int rc = check_hip(hipMalloc(&s->lmem, cap * (11 * sizeof(float) + 4 * sizeof(float) + 4 * sizeof(double))),
+  int rc = check_hip(hipMalloc(&s->lmem, cap * (11 * sizeof(float) + 4 * sizeof(float) + 8 * sizeof(double))),
                      "hipMalloc learned buffers");
   if (rc) return rc;
-  s->gx = (double *)s->lmem;
-  s->feat = (float *)(s->gx + 4 * cap);
+  s->gxg = (double *)s->lmem;
+  s->gxr = s->gxg + 4 * cap;
+  s->feat = (float *)(s->gxr + 4 * cap);
   s->y = s->feat + 11 * cap;
   return 0;
 }
@@ -362,11 +363,14 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   return check_launch("koral_hip_implicit_batch");
 }
 
-int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, double *uu, double *pp,
-                                     const double *g20, const double *G20, const double *dt, double global_time,
-                                     int maxiter, int *flags, int *info, void *stream)
+int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
+                                     double *uu, double *pp, const double *g20, const double *G20,
+                                     const double *dt, double global_time, int maxiter, int *flags, int *info,
+                                     void *stream)
 {
-  if (check_params(P) || n < 0 || !m) return set_error(KORAL_E_ARG, "implicit_batch_learned: bad arguments");
+  if (check_params(P) || n < 0 || !gas) return set_error(KORAL_E_ARG, "implicit_batch_learned: bad arguments");
+  if (koral_hip_mlp_kind(gas) != 0 || (rad && koral_hip_mlp_kind(rad) != 1))
+    return set_error(KORAL_E_ARG, "implicit_batch_learned: networks of the wrong kind");
   if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
   if (n == 0) return 0;
   if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "implicit batch larger than 2^31 cells");
@@ -375,16 +379,20 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *m, lo
   int rc = acquire_set(n, s, &set);
   if (rc) return rc;
   rc = learned_buffers(set);
-  if (!rc) rc = koral_hip_mlp_features(P, m, n, pp, dt, set->feat, stream);
-  if (!rc) rc = koral_hip_mlp_infer(m, set->feat, n, set->y, stream);
-  if (!rc) rc = koral_hip_mlp_guess(m, n, pp, set->y, set->gx, stream);
+  if (!rc) rc = koral_hip_mlp_features(P, gas, n, pp, dt, set->feat, stream);
+  if (!rc) rc = koral_hip_mlp_infer(gas, set->feat, n, set->y, stream);
+  if (!rc) rc = koral_hip_mlp_guess(gas, n, P->nv, pp, set->y, set->gxg, stream);
+  if (!rc && rad) rc = koral_hip_mlp_features(P, rad, n, pp, dt, set->feat, stream);
+  if (!rc && rad) rc = koral_hip_mlp_infer(rad, set->feat, n, set->y, stream);
+  if (!rc && rad) rc = koral_hip_mlp_guess(rad, n, P->nv, pp, set->y, set->gxr, stream);
+  const double *gxr = rad ? set->gxr : nullptr;
   if (!rc) {
     if (P->mag) {
       BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gx, maxiter);
+      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gxg, gxr, maxiter);
     } else {
       BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gx, maxiter);
+      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gxg, gxr, maxiter);
     }
   }
   int rc2 = release_set(set, s);

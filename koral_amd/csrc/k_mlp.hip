@@ -1,9 +1,11 @@
 // Learned initial guess for the implicit radiation solve: a small MLP
 // (11 -> 128 -> 128 -> 128 -> 4, GELU in tanh form) that maps the
 // dimensionless local state of a cell to corrections of the plain Newton
-// start (u, u~^1..3).  Training is FP32 (forward, backward and Adam kernels
-// below, deterministic reductions); inference runs the weights in BF16 on
-// the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+// start.  A gas network corrects (u, u~^1..3), a radiation network
+// (E, F~^1..3); the solve picks one per cell.  Training is FP32 (forward,
+// backward and Adam kernels below, deterministic reductions); inference runs
+// the weights in BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation).
 //
 // Parameters: one FP32 vector W1[128][11], b1, W2[128][128], b2, W3[128][128],
 // b3, W4[4][128], b4 (row-major W[out][in]).  Normalisation: feature mean and
@@ -11,8 +13,9 @@
 //
 // Features of a cell: log10 rho, log10(u/rho), log10(E/u), gas u~^i (3),
 // radiation-frame u~^i (3), log10(dt kappa_abs), log10(dt kappa_es).
-// Targets: log(u*/u0), u~*^i - u~0^i for the converged energy-equation
-// lab-frame state (u*, u~*) and the initial state (u0, u~0).
+// Targets between the converged energy-equation lab-frame state (x*) and the
+// initial state (x0) of the network's four primitives: log(x*_0 / x0_0) and
+// asinh((x*_i - x0_i) / VSCALE) for the three velocity components.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <mutex>
@@ -31,6 +34,9 @@ constexpr int OW1 = 0, OB1 = OW1 + NH * NIN, OW2 = OB1 + NH, OB2 = OW2 + NH * NH
 constexpr int NNORM = 2 * NIN + 2 * NOUT;  // feature mean, std, target mean, std
 constexpr int BMAX = 4096;                 // largest training batch
 constexpr float C_GELU = 0.7978845608028654f;
+// velocity-like targets are asinh(delta / VSCALE): linear below VSCALE,
+// logarithmic above, so changes from 1e-10 to 1 carry comparable weight
+constexpr double VSCALE = 1e-6;
 
 __device__ __forceinline__ float gelu(float z)
 {
@@ -74,22 +80,15 @@ __global__ void __launch_bounds__(256) k_features(koral_params P, long n, const 
   }
 }
 
-__global__ void __launch_bounds__(256) k_targets(long n, int nv, const double *__restrict__ p0,
-                                                 const double *__restrict__ p1, const float *__restrict__ norm,
-                                                 float *__restrict__ t)
+__global__ void __launch_bounds__(256) k_targets(long n, int base, const double *__restrict__ p0,
+                                                 const double *__restrict__ p1, float *__restrict__ t)
 {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  float v[NOUT];
-  v[0] = (float)log(p1[1 * n + c] / p0[1 * n + c]);
+  t[c] = (float)log(p1[base * n + c] / p0[base * n + c]);
 #pragma unroll
-  for (int i = 1; i < NOUT; i++) v[i] = (float)(p1[(1 + i) * n + c] - p0[(1 + i) * n + c]);
-#pragma unroll
-  for (int i = 0; i < NOUT; i++) {
-    if (norm) v[i] = (v[i] - norm[2 * NIN + i]) / norm[2 * NIN + NOUT + i];
-    t[i * n + c] = v[i];
-  }
-  (void)nv;
+  for (int i = 1; i < NOUT; i++)
+    t[i * n + c] = (float)asinh((p1[(base + i) * n + c] - p0[(base + i) * n + c]) / VSCALE);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,21 +322,24 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const unsigned short *__restr
 
 // guess (u, u~^i) = plain start corrected by the de-normalised outputs;
 // non-finite or non-positive guesses fall back to the plain start
-__global__ void __launch_bounds__(256) k_guess(long n, const double *__restrict__ pp, const float *__restrict__ y,
-                                               const float *__restrict__ norm, double *__restrict__ gx)
+__global__ void __launch_bounds__(256) k_guess(long n, int base, const double *__restrict__ pp,
+                                               const float *__restrict__ y, const float *__restrict__ norm,
+                                               double *__restrict__ gx)
 {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double t[NOUT];
 #pragma unroll
   for (int j = 0; j < NOUT; j++) t[j] = (double)y[j * n + c] * norm[2 * NIN + NOUT + j] + norm[2 * NIN + j];
-  double g0 = pp[1 * n + c] * exp(t[0]);
-  double g1 = pp[2 * n + c] + t[1], g2 = pp[3 * n + c] + t[2], g3 = pp[4 * n + c] + t[3];
-  bool ok = isfinite(g0) && g0 > 0. && isfinite(g1) && isfinite(g2) && isfinite(g3);
-  gx[0 * n + c] = ok ? g0 : pp[1 * n + c];
-  gx[1 * n + c] = ok ? g1 : pp[2 * n + c];
-  gx[2 * n + c] = ok ? g2 : pp[3 * n + c];
-  gx[3 * n + c] = ok ? g3 : pp[4 * n + c];
+  const double x0 = pp[base * n + c], x1 = pp[(base + 1) * n + c], x2 = pp[(base + 2) * n + c],
+               x3 = pp[(base + 3) * n + c];
+  const double g0 = x0 * exp(t[0]);
+  const double g1 = x1 + VSCALE * sinh(t[1]), g2 = x2 + VSCALE * sinh(t[2]), g3 = x3 + VSCALE * sinh(t[3]);
+  const bool ok = isfinite(g0) && g0 > 0. && isfinite(g1) && isfinite(g2) && isfinite(g3);
+  gx[0 * n + c] = ok ? g0 : x0;
+  gx[1 * n + c] = ok ? g1 : x1;
+  gx[2 * n + c] = ok ? g2 : x2;
+  gx[3 * n + c] = ok ? g3 : x3;
 }
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -348,6 +350,7 @@ inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 // the network object
 // ---------------------------------------------------------------------------
 struct koral_hip_mlp {
+  int kind = 0;  // 0: gas primitives (u, u~^i), 1: radiation primitives (E, F~^i)
   float *th = nullptr, *m = nullptr, *v = nullptr, *g = nullptr, *norm = nullptr;
   unsigned short *wp = nullptr;
   bool packed = false;
@@ -426,10 +429,11 @@ extern "C" {
 
 int koral_hip_mlp_nparam(void) { return NPARAM; }
 
-int koral_hip_mlp_create(koral_hip_mlp **out)
+int koral_hip_mlp_create(int kind, koral_hip_mlp **out)
 {
-  if (!out) return set_error(KORAL_E_ARG, "mlp_create: null");
+  if (!out || kind < 0 || kind > 1) return set_error(KORAL_E_ARG, "mlp_create: bad arguments");
   auto *m = new koral_hip_mlp();
+  m->kind = kind;
   int rc = 0;
   float **vecs[4] = {&m->th, &m->m, &m->v, &m->g};
   for (int k = 0; k < 4 && !rc; k++) rc = m->alloc(vecs[k], NPARAM);
@@ -496,12 +500,15 @@ int koral_hip_mlp_features(const koral_params *P, const koral_hip_mlp *m, long n
   return check_launch("mlp_features");
 }
 
-int koral_hip_mlp_targets(const koral_hip_mlp *m, long n, int nv, const double *pp0, const double *pp1,
-                          float *targ, void *stream)
+int koral_hip_mlp_kind(const koral_hip_mlp *m) { return m ? m->kind : -1; }
+
+int koral_hip_mlp_targets(int kind, long n, int nv, const double *pp0, const double *pp1, float *targ,
+                          void *stream)
 {
-  if (n < 0 || !pp0 || !pp1 || !targ) return set_error(KORAL_E_ARG, "mlp_targets");
+  if (n < 0 || !pp0 || !pp1 || !targ || kind < 0 || kind > 1 || (kind == 1 && nv < 10))
+    return set_error(KORAL_E_ARG, "mlp_targets: bad arguments");
   if (n == 0) return 0;
-  k_targets<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, nv, pp0, pp1, m ? m->norm : nullptr, targ);
+  k_targets<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, kind == 0 ? 1 : nv - 4, pp0, pp1, targ);
   return check_launch("mlp_targets");
 }
 
@@ -556,11 +563,13 @@ int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, v
   return check_launch("mlp_infer");
 }
 
-int koral_hip_mlp_guess(koral_hip_mlp *m, long n, const double *pp, const float *Y, double *gx, void *stream)
+int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, const float *Y, double *gx,
+                        void *stream)
 {
-  if (!m || n < 0 || !pp || !Y || !gx) return set_error(KORAL_E_ARG, "mlp_guess: bad arguments");
+  if (!m || n < 0 || !pp || !Y || !gx || (m->kind == 1 && nv < 10))
+    return set_error(KORAL_E_ARG, "mlp_guess: bad arguments");
   if (n == 0) return 0;
-  k_guess<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, pp, Y, m->norm, gx);
+  k_guess<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, m->kind == 0 ? 1 : nv - 4, pp, Y, m->norm, gx);
   return check_launch("mlp_guess");
 }
 

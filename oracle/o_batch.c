@@ -121,7 +121,7 @@ void oracle_ff_Rtt_batch(const oracle_params *P, long n, const double *pp, const
    {iters, attempt, whichprim, whicheq, whichframe, total_iters} */
 void oracle_implicit_batch_guess(const oracle_params *P, long n, double *uu, double *pp, const double *g20,
                                  const double *G20, const double *dt, double global_time, int *flags, int *info,
-                                 int nthreads, const double *gx, int maxiter)
+                                 int nthreads, const double *gxg, const double *gxr, int maxiter)
 {
   int nv = P->nv;
   if (nthreads < 1) nthreads = 1;
@@ -131,7 +131,7 @@ void oracle_implicit_batch_guess(const oracle_params *P, long n, double *uu, dou
     o_fill_geom(&g, g20 + 20 * i, G20 + 20 * i);
     o_imp_info inf;
     o_solve_implicit_lab_guess(P, uu + nv * i, pp + nv * i, &g, dt[i], global_time, flags + O_NFLAGS * i, &inf,
-                               gx ? gx + 4 * i : NULL, maxiter);
+                               gxg ? gxg + 4 * i : NULL, gxr ? gxr + 4 * i : NULL, maxiter);
     int *o = info + 6 * i;
     o[0] = inf.iters;
     o[1] = inf.attempt;
@@ -146,7 +146,7 @@ void oracle_implicit_batch(const oracle_params *P, long n, double *uu, double *p
                            const double *G20, const double *dt, double global_time, int *flags, int *info,
                            int nthreads)
 {
-  oracle_implicit_batch_guess(P, n, uu, pp, g20, G20, dt, global_time, flags, info, nthreads, NULL, 0);
+  oracle_implicit_batch_guess(P, n, uu, pp, g20, G20, dt, global_time, flags, info, nthreads, NULL, NULL, 0);
 }
 
 /* ---- grid-level entry points (o_grid.c) ---- */

@@ -154,10 +154,11 @@ int o_solve_implicit_lab(const oracle_params *P, double *uu, double *pp, const o
                          double dt, double global_time, int *flags, o_imp_info *info);
 int o_solve_implicit_4dprim_from(const oracle_params *P, const double *uu00, const double *pp00,
                                  const o_geom *g, double dt, const int *params, double *pp,
-                                 double global_time, int *iters_out, const double *x0, int maxiter);
+                                 double global_time, int *iters_out, const double *xg, const double *xr,
+                                 int maxiter);
 int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, const o_geom *g,
                                double dt, double global_time, int *flags, o_imp_info *info,
-                               const double *gx, int maxiter);
+                               const double *gxg, const double *gxr, int maxiter);
 
 /* --- batched entry points used by tests and the CPU baseline --- */
 int oracle_params_size(void);
@@ -181,7 +182,7 @@ void oracle_implicit_batch(const oracle_params *P, long n, double *uu, double *p
                            double global_time, int *flags, int *info, int nthreads);
 void oracle_implicit_batch_guess(const oracle_params *P, long n, double *uu, double *pp, const double *g20,
                                  const double *G20, const double *dt, double global_time, int *flags, int *info,
-                                 int nthreads, const double *gx, int maxiter);
+                                 int nthreads, const double *gxg, const double *gxr, int maxiter);
 int oracle_inverse44(const double *a, double *ia);
 
 #ifdef __cplusplus

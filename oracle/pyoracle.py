@@ -41,7 +41,7 @@ def lib():
         L.oracle_implicit_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                             _ip, _ip, ctypes.c_int]
         L.oracle_implicit_batch_guess.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
-                                                  _ip, _ip, ctypes.c_int, _dp, ctypes.c_int]
+                                                  _ip, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.oracle_inverse44.argtypes = [_dp, _dp]
         L.oracle_inverse44.restype = ctypes.c_int
         _lib = L
@@ -130,27 +130,33 @@ def ff_Rtt(P, pp, g20, G20):
     return out
 
 
-def implicit(P, uu, pp, g20, G20, dt, global_time=0.0, nthreads=1, gx=None, maxiter=0):
+def implicit(P, uu, pp, g20, G20, dt, global_time=0.0, nthreads=1, gx=None, maxiter=0, gxr=None):
     """implicit_lab_rad_source_term on independent cells.
     Returns (uu, pp, flags[n,7], info[n,6]) with info columns
     iters, attempt, whichprim, whicheq, whichframe, total_iters.
-    gx [n, 4] (u, u~^1..3), when given, is the start of a first attempt on
-    the gas primitives (energy equation, lab frame, at most maxiter
-    iterations when maxiter > 0), reported as attempt 6 when it succeeds."""
+    gx [n, 4] (u, u~^1..3), when given, is the start of a first attempt
+    (energy equation, lab frame, at most maxiter iterations when maxiter > 0)
+    on the gas primitives, or, with gxr [n, 4] (E, F~^1..3) given, on the
+    radiation primitives from gxr where the completed gas start asks for
+    them; reported as attempt 6 when it succeeds."""
     uu = np.array(uu, dtype=np.float64, order="C")
     pp = np.array(pp, dtype=np.float64, order="C")
     n = pp.shape[0]
     flags = np.zeros((n, 7), np.int32)
     info = np.zeros((n, 6), np.int32)
     dt = np.ascontiguousarray(dt, dtype=np.float64)
-    gxp = None
-    if gx is not None:
-        gx = np.ascontiguousarray(gx, dtype=np.float64)
-        assert gx.shape == (n, 4)
-        gxp = _d(gx)
+    ptrs = []
+    for a in (gx, gxr):
+        if a is None:
+            ptrs.append(None)
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (n, 4)
+        ptrs.append(a)
     lib().oracle_implicit_batch_guess(_P(P), n, _d(uu), _d(pp), _d(np.ascontiguousarray(g20)),
                                       _d(np.ascontiguousarray(G20)), _d(dt), global_time, _i(flags), _i(info),
-                                      nthreads, gxp, int(maxiter))
+                                      nthreads, _d(ptrs[0]) if ptrs[0] is not None else None,
+                                      _d(ptrs[1]) if ptrs[1] is not None else None, int(maxiter))
     return uu, pp, flags, info
 
 

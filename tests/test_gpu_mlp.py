@@ -122,107 +122,125 @@ def test_bf16_inference_matches_reference(dev):
     assert err32 < 5e-2
 
 
-def test_learned_guess_solve(dev, oracle):
-    """Train on pairs from the plain solver, then solve a fresh batch from the
-    learned guess: every cell the plain chain solves is solved, cells solved
-    by both with the energy equation in the lab frame agree, and the mean
-    number of Newton iterations drops."""
+def test_targets_and_guess_match_reference(dev, oracle):
+    """Target kernel of both kinds against the numpy reference, and the guess
+    kernel as its inverse (outputs set to the standardised targets)."""
     import torch
+    import mlp as M
     from koral_amd import params as KP, problems as PR, cell
     from koral_amd import learn as LN
     P = KP.params_implicit_batch()
-    # training pairs
-    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 17, seed=21)
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 4096, seed=8)
+    uu = oracle.p2u(P, pp, g20, G20)
+    _, p1, _, info = oracle.implicit(P, uu, pp, g20, G20, dt, nthreads=8)
+    ok = info[:, 1] >= 0
+    pp, p1 = pp[ok], p1[ok]
+    P0, P1 = cell.to_dev(pp), cell.to_dev(p1)
+    for kind, base in ((LN.GAS, 1), (LN.RAD, P.ee)):
+        T = LN.raw_targets(P0, P1, kind).cpu().numpy().T
+        ref = M.targets(pp, p1, base)
+        err = np.abs(T - ref) / (np.abs(ref) + 1e-3)
+        net = LN.Net(kind=kind)
+        norm = np.concatenate([np.zeros(11), np.ones(11), [0.1, -0.2, 0.3, 0.], [2., 3., 4., 5.]]).astype(np.float32)
+        net.set(LN.init_params(0), norm)
+        Y = ((torch.from_numpy(ref.T.copy()).float().to(dev) - torch.from_numpy(norm[22:26, None]).to(dev))
+             / torch.from_numpy(norm[26:30, None]).to(dev)).contiguous()
+        gx = net.guess(P0, Y).cpu().numpy().T
+        gerr = np.abs(gx - p1[:, base:base + 4]) / (np.abs(p1[:, base:base + 4]) + 1e-6)
+        _say("kind %d: targets max rel %.2e, guess from targets max rel %.2e" % (kind, err.max(), gerr.max()))
+        assert err.max() < 1e-5
+        assert gerr.max() < 1e-5
+
+
+def _trained_pair(dev, P, n_train, seed, schedule):
+    import torch
+    from koral_amd import problems as PR, cell
+    from koral_amd import learn as LN
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n_train, seed=seed)
     Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
     U = cell.p2u(P, Pp, g, G)
     P0 = Pp.clone()
     _, info = cell.implicit(P, U, Pp, g, G, dtd)
-    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
-    norm = LN.normalisation(F, T)
-    Fs, Ts = LN.standardise(F, T, norm)
-    net = LN.Net(seed=0)
-    net.set(LN.init_params(0), norm)
-    hist = net.fit(Fs, Ts, epochs=40, lr=1e-3)
-    _say("training pairs %d, loss per epoch %s" % (F.shape[1], ["%.4f" % h for h in hist]))
-    assert hist[-1] < hist[0]
-    # fresh batch: plain and learned
+    F, Tg, Tr = LN.training_pairs(P, P0, Pp, dtd, info)
+    gas, hg = LN.train(P, F, Tg, LN.GAS, schedule=schedule)
+    rad, hr = LN.train(P, F, Tr, LN.RAD, schedule=schedule)
+    return gas, rad, hg, hr, F.shape[1]
+
+
+def test_learned_guess_solve(dev, oracle):
+    """Train both networks on pairs from the plain solver, then solve a fresh
+    batch from the learned guess: every cell the plain chain solves is solved,
+    cells the learned attempt leaves to the chain get exactly the plain
+    result, cells solved by both from the gas primitives agree to the solver
+    tolerance, and the mean number of Newton iterations drops."""
+    import torch
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd import learn as LN
+    P = KP.params_implicit_batch()
+    gas, rad, hg, hr, npairs = _trained_pair(dev, P, 1 << 18, 21, [(1e-3, 12), (3e-4, 6), (1e-4, 3)])
+    _say("training pairs %d, loss gas %s rad %s" % (npairs, ["%.3f" % h for h in hg], ["%.3f" % h for h in hr]))
+    assert hg[-1] < hg[0] and hr[-1] < hr[0]
     pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 15, seed=22)
     Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
     U0 = cell.p2u(P, Pp, g, G)
     Ua, Pa = U0.clone(), Pp.clone()
     _, ia = cell.implicit(P, Ua, Pa, g, G, dtd)
-    Ub, Pb = U0.clone(), Pp.clone()
-    _, ib = LN.implicit_learned(P, net, Ub, Pb, g, G, dtd)
-    ia, ib = ia.cpu().numpy(), ib.cpu().numpy()
-    pa, pb = cell.to_host(Pa), cell.to_host(Pb)
-    ok_a, ok_b = ia[1] >= 0, ib[1] >= 0
-    assert np.all(ok_b[ok_a]), "the learned attempt may only add solved cells"
-    learned = ib[1] == 6
-    # cells the plain chain solved with the same system (energy equation, lab
-    # frame, gas primitives) from the old state
-    both = learned & (ia[1] == 0) & (ia[2] == 2)
-    same = (ib[1] == ia[1]) & ok_a
-    du = np.abs(pa[:, 1] - pb[:, 1]) / np.abs(pa[:, 1])
-    dv = np.abs(pa[:, 2:5] - pb[:, 2:5]).max(1) / (1. + np.abs(pa[:, 2:5]).max(1))
-    it_a, it_b = ia[5].mean(), ib[5].mean()
-    _say("cells %d: plain solved %d, learned solved %d (%d by the learned attempt); mean iterations "
-         "plain %.3f learned %.3f (x%.2f)" %
-         (len(ok_a), ok_a.sum(), ok_b.sum(), learned.sum(), it_a, it_b, it_a / it_b))
-    _say("same system from both starts (%d cells): |du|/u median %.2e p99 %.2e max %.2e; "
-         "velocity max %.2e" % (both.sum(), np.median(du[both]), np.quantile(du[both], 0.99), du[both].max(),
-                                dv[both].max()))
-    for budget in (24, 16, 10):
-        Uc, Pc = U0.clone(), Pp.clone()
-        _, ic = LN.implicit_learned(P, net, Uc, Pc, g, G, dtd, maxiter=budget)
-        ic = ic.cpu().numpy()
-        _say("learned attempt budget %d: solved %d (%d learned), mean iterations %.3f" %
-             (budget, (ic[1] >= 0).sum(), (ic[1] == 6).sum(), ic[5].mean()))
-        assert np.all(ic[1][ok_a] >= 0)
-    # cells the learned attempt did not solve follow the plain chain exactly
-    assert np.array_equal(pa[same], pb[same])
-    assert learned.sum() > 0.5 * len(ok_a)
-    # two converged Newton solves of one system agree to the solver tolerance
-    assert np.quantile(du[both], 0.99) < 1e-4 and np.quantile(dv[both], 0.99) < 1e-4
-    assert it_b < it_a
+    ia = ia.cpu().numpy()
+    pa = cell.to_host(Pa)
+    ok_a = ia[1] >= 0
+    res = {}
+    for name, r, budget in (("gas", None, 0), ("gas+rad", rad, 0), ("gas+rad", rad, 16), ("gas+rad", rad, 10)):
+        Ub, Pb = U0.clone(), Pp.clone()
+        _, ib = LN.implicit_learned(P, gas, r, Ub, Pb, g, G, dtd, maxiter=budget)
+        ib = ib.cpu().numpy()
+        pb = cell.to_host(Pb)
+        ok_b = ib[1] >= 0
+        learned = ib[1] == 6
+        _say("%s budget %d: plain solved %d, learned solved %d (%d by the learned attempt, %d on radiation "
+             "primitives); mean iterations plain %.3f learned %.3f (x%.2f)" %
+             (name, budget, ok_a.sum(), ok_b.sum(), learned.sum(), (learned & (ib[2] == 1)).sum(),
+              ia[5].mean(), ib[5].mean(), ia[5].mean() / ib[5].mean()))
+        assert np.all(ok_b[ok_a]), "the learned attempt may only add solved cells"
+        same = (ib[1] == ia[1]) & ok_a
+        assert np.array_equal(pa[same], pb[same])
+        both = learned & (ia[1] == 0) & (ia[2] == 2) & (ib[2] == 2)
+        du = np.abs(pa[:, 1] - pb[:, 1]) / np.abs(pa[:, 1])
+        _say("  same system from both starts (%d cells): |du|/u median %.2e p99 %.2e" %
+             (both.sum(), np.median(du[both]), np.quantile(du[both], 0.99)))
+        res[(name, budget)] = ib[5].mean()
+    assert res[("gas+rad", 0)] < ia[5].mean()
 
 
 def test_learned_solve_matches_oracle(dev, oracle):
     """The device solve with the learned first attempt against the oracle's
-    solve from the same start (downloaded from the network kernels): same
+    solve from the same starts (downloaded from the network kernels): same
     attempt indices and iteration counts, primitives to the parity bar of the
     plain implicit solve."""
     import torch
     from koral_amd import params as KP, problems as PR, cell
     from koral_amd import learn as LN
     P = KP.params_implicit_batch()
-    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 16, seed=41)
-    Pp, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
-    U = cell.p2u(P, Pp, g, G)
-    P0 = Pp.clone()
-    _, info = cell.implicit(P, U, Pp, g, G, dtd)
-    F, T = LN.training_pairs(P, P0, Pp, dtd, info)
-    norm = LN.normalisation(F, T)
-    Fs, Ts = LN.standardise(F, T, norm)
-    net = LN.Net(seed=1)
-    net.set(LN.init_params(1), norm)
-    net.fit(Fs, Ts, epochs=3, lr=1e-3)
+    gas, rad, _, _, _ = _trained_pair(dev, P, 1 << 16, 41, [(1e-3, 3)])
     n = 3000
     pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=42)
     uu = oracle.p2u(P, pp, g20, G20)
     Pd, Ud, gd, Gd = cell.to_dev(pp), cell.to_dev(uu), cell.to_dev(g20), cell.to_dev(G20)
     dtd = torch.from_numpy(dt).to(dev)
-    gx = net.guess(Pd, net.infer(net.features(P, Pd, dtd))).cpu().numpy().T.copy()
-    for budget in (0, 12):
+    gxg = gas.start(P, Pd, dtd).cpu().numpy().T.copy()
+    gxr = rad.start(P, Pd, dtd).cpu().numpy().T.copy()
+    for r, budget in ((None, 0), (rad, 0), (rad, 12)):
         U1, P1 = Ud.clone(), Pd.clone()
-        fl, info = LN.implicit_learned(P, net, U1, P1, gd, Gd, dtd, maxiter=budget)
+        fl, info = LN.implicit_learned(P, gas, r, U1, P1, gd, Gd, dtd, maxiter=budget)
         info = info.cpu().numpy().T
         po = cell.to_host(P1)
-        uo_r, po_r, fl_r, info_r = oracle.implicit(P, uu, pp, g20, G20, dt, gx=gx, maxiter=budget)
+        uo_r, po_r, fl_r, info_r = oracle.implicit(P, uu, pp, g20, G20, dt, gx=gxg, maxiter=budget,
+                                                   gxr=gxr if r is not None else None, nthreads=8)
         same = np.all(info == info_r, axis=1)
         ok = same & (info[:, 1] >= 0)
         rel = np.abs(po[ok] - po_r[ok]) / (np.abs(po[ok]) + np.abs(po_r[ok]) + 1e-300)
-        _say("budget %d: %d cells, learned attempt %d, info identical %.4f, max rel p %.2e" %
-             (budget, n, (info[:, 1] == 6).sum(), same.mean(), rel.max()))
+        _say("%s budget %d: %d cells, learned attempt %d (%d radiation primitives), info identical %.4f, "
+             "max rel p %.2e" % ("gas" if r is None else "gas+rad", budget, n, (info[:, 1] == 6).sum(),
+                                 ((info[:, 1] == 6) & (info[:, 2] == 1)).sum(), same.mean(), rel.max()))
         assert (info[:, 1] == 6).sum() > 0
         assert same.mean() > 0.99
         assert rel.max() < 1e-9
