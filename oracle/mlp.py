@@ -11,9 +11,10 @@ W2[128][128], b2[128], W3[128][128], b3[128], W4[4][128], b4[4]
 Features of a cell (standardised with the training set's mean and std):
   log10 rho, log10(u/rho), log10(E/u), the three VELR components of the gas
   and of the radiation frame, log10(dt kappa_abs), log10(dt kappa_es)
-Targets (standardised with the training set's std): log(u*/u0) and the
-three differences u~*^i - u~0^i between the converged state of the
-energy-equation lab-frame solve and the initial state.
+Targets (standardised with the training set's mean and std) between the
+converged state of the energy-equation lab-frame solve (x*) and the initial
+state (x0) of the network's four primitives (gas: u, u~^i; radiation: E,
+F~^i): log(x*_0 / x0_0) and asinh((x*_i - x0_i) / 1e-6).
 """
 import numpy as np
 
@@ -114,9 +115,22 @@ def raw_features(P, pp, dt, kappa_abs, kappa_es):
     return f
 
 
-def targets(pp0, pp1):
-    """Raw targets [n, 4] from initial and converged primitives."""
+VSCALE = 1e-6
+
+
+def targets(pp0, pp1, base=1):
+    """Raw targets [n, 4] between initial and converged primitives of the
+    network's kind: base 1 (gas: u, u~^i) or EE (radiation: E, F~^i):
+    log(x1_0 / x0_0) and asinh((x1_k - x0_k) / VSCALE)."""
     t = np.empty((pp0.shape[0], NOUT))
-    t[:, 0] = np.log(pp1[:, 1] / pp0[:, 1])
-    t[:, 1:4] = pp1[:, 2:5] - pp0[:, 2:5]
+    t[:, 0] = np.log(pp1[:, base] / pp0[:, base])
+    t[:, 1:4] = np.arcsinh((pp1[:, base + 1:base + 4] - pp0[:, base + 1:base + 4]) / VSCALE)
     return t
+
+
+def guess(pp0, t, base=1):
+    """Newton start [n, 4] from raw targets t (inverse of targets)."""
+    g = np.empty((pp0.shape[0], NOUT))
+    g[:, 0] = pp0[:, base] * np.exp(t[:, 0])
+    g[:, 1:4] = pp0[:, base + 1:base + 4] + VSCALE * np.sinh(t[:, 1:4])
+    return g

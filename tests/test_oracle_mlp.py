@@ -59,5 +59,13 @@ def test_features_and_targets():
     np.testing.assert_allclose(f[:, 10], [0., 0.], atol=1e-15)
     pp1 = pp.copy()
     pp1[:, 1] *= np.e
-    pp1[:, 2:5] += 1.
-    np.testing.assert_allclose(M.targets(pp, pp1), [[1., 1., 1., 1.]] * 2)
+    pp1[:, 2:5] += 1e-6
+    pp1[:, 10:13] -= 2e-6
+    pp1[:, 9] /= np.e ** 2
+    s1, s2 = np.arcsinh(1.), np.arcsinh(2.)
+    np.testing.assert_allclose(M.targets(pp, pp1), [[1., s1, s1, s1]] * 2, rtol=1e-6)
+    np.testing.assert_allclose(M.targets(pp, pp1, base=9), [[-2., -s2, -s2, -s2]] * 2, rtol=1e-6)
+    # the start follows back from the targets
+    for base in (1, 9):
+        np.testing.assert_allclose(M.guess(pp, M.targets(pp, pp1, base), base), pp1[:, base:base + 4],
+                                   rtol=1e-9, atol=1e-15)
