@@ -396,8 +396,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cells", type=int, default=1 << 22)
     ap.add_argument("--train-cells", type=int, default=1 << 23)
-    ap.add_argument("--epochs", type=int, default=2)
-    ap.add_argument("--learned-maxiter", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--learned-maxiter", type=int, default=16)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)
