@@ -232,6 +232,11 @@ int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt);
    implicit failures, hd fixup requests, rad fixup requests, entropy
    inversions} may be NULL. */
 int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstepdenmax, int *counters);
+/* nsteps RK2IMEX steps of ko.c's loop (ko.c:399-570) with the time kept on the device: each
+   step runs at (*t, *dt), then t += dt and dt = 1 / tstepdenmax of that step, as the
+   reference's loop does, with no host round trip between steps.  On return *t, *dt are the
+   time and step of the next step; counters[6] (may be NULL) sum the step counters. */
+int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, int *counters);
 
 #ifdef __cplusplus
 }

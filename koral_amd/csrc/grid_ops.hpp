@@ -539,11 +539,12 @@ struct OpUpdate {
   const double *p;
   double *u;
   Flx F;
-  double dt;
+  const double *clk;  // device clock {t, dt}
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   KD_INL void operator()(long i) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
+    const double dt = clk[1];
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
     int ix, iy, iz;
     b.at(i, ix, iy, iz);
@@ -716,11 +717,13 @@ struct OpImplicit {
   DGrid G;
   double *u, *p;
   signed char *fl;
-  double dt, t;
+  const double *clk;  // device clock {t, dt}; the solve's time step is dt * fac
+  double fac;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   KD_INL Cnt3 operator()(long i) const
   {
     constexpr int NV = Lay<MAG, 1>::NV;
+    const double t = clk[0], dt = clk[1] * fac;
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
     int ix, iy, iz;
     b.at(i, ix, iy, iz);
@@ -745,16 +748,38 @@ struct OpImplicit {
 };
 
 // RK2IMEX linear combination over the domain: out = a x + b y (+ c z)
+// A coefficient of the step algebra as the expression of ko.c:444-570 it
+// stands for, evaluated on the device from the clock's dt with the same
+// operations the host would use: c, c / (dt * gamma), c / dt, dt * c, dt / c.
+struct Coef {
+  enum Kind { CONST, C_OVER_DTG, C_OVER_DT, DT_TIMES_C, DT_OVER_C } kind;
+  double c;
+  KD_HD double eval(double dt, double gamma) const
+  {
+    switch (kind) {
+    case C_OVER_DTG: return c / (dt * gamma);
+    case C_OVER_DT: return c / dt;
+    case DT_TIMES_C: return dt * c;
+    case DT_OVER_C: return dt / c;
+    default: return c;
+    }
+  }
+};
+
 struct OpComb {
   static constexpr int kind = K_COMB;
   DGrid G;
   int nv;
   double *out;
-  double a, b, c;
+  Coef ca, cb, cc;
   const double *x, *y, *z;
+  const double *clk;
+  double gamma;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz * nv; }
   KD_INL void operator()(long i) const
   {
+    const double dt = clk[1];
+    const double a = ca.eval(dt, gamma), b = cb.eval(dt, gamma), c = cc.eval(dt, gamma);
     long nd = (long)G.nx * G.ny * G.nz;
     int iv = (int)(i / nd);
     long r = i % nd;
@@ -786,6 +811,7 @@ struct GridBufs {
   int *mark;
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
   double *tsd;    // time-step density max (one double)
+  double *clock;  // {t, dt} of the operators (device scalars)
   int *counters;  // step counters [6]
   unsigned *imp_list[2][2];  // work lists of the implicit passes (device executor)
   unsigned *imp_ctr;         // their counters
@@ -880,7 +906,7 @@ void tstepden(Ex &ex, const GridBufs &s)
 // op_explicit: ghosts, wave speeds (tsd accumulates the max), face fluxes,
 // flux-CT, update
 template <class Ex>
-void explicit_op(Ex &ex, const GridBufs &s, double dt)
+void explicit_op(Ex &ex, const GridBufs &s)
 {
   const DGrid &G = s.G;
   const koral_params &P = s.P;
@@ -924,7 +950,7 @@ void explicit_op(Ex &ex, const GridBufs &s, double dt)
   }
   dispatch(P, [&](auto t) {
     using T = decltype(t);
-    ex.each(OpUpdate<T::MAG, T::RAD>{P, G, s.p, s.u, F, dt});
+    ex.each(OpUpdate<T::MAG, T::RAD>{P, G, s.p, s.u, F, s.clock});
   });
 }
 
@@ -1025,49 +1051,54 @@ void do_correct(Ex &ex, const GridBufs &s)
 }
 
 template <class Ex>
-void implicit_op(Ex &ex, const GridBufs &s, double t, double dt)
+void implicit_op(Ex &ex, const GridBufs &s, double fac)
 {
   if (!s.P.rad) return;
   if (s.P.mag)
-    ex.implicit_cells(OpImplicit<1>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+    ex.implicit_cells(OpImplicit<1>{s.P, s.G, s.u, s.p, s.fl, s.clock, fac}, s.counters);
   else
-    ex.implicit_cells(OpImplicit<0>{s.P, s.G, s.u, s.p, s.fl, dt, t}, s.counters);
+    ex.implicit_cells(OpImplicit<0>{s.P, s.G, s.u, s.p, s.fl, s.clock, fac}, s.counters);
   fixup(ex, s, F_RADIMPFIXUP);
 }
 
 template <class Ex>
-void comb(Ex &ex, const GridBufs &s, double *out, double a, const double *x, double b = 0.,
-          const double *y = nullptr, double c = 0., const double *z = nullptr)
+void comb(Ex &ex, const GridBufs &s, double gamma, double *out, Coef a, const double *x,
+          Coef b = {Coef::CONST, 0.}, const double *y = nullptr, Coef c = {Coef::CONST, 0.},
+          const double *z = nullptr)
 {
-  ex.each(OpComb{s.G, s.P.nv, out, a, b, c, x, y, z});
+  ex.each(OpComb{s.G, s.P.nv, out, a, b, c, x, y, z, s.clock, gamma});
 }
 
-// body of one RK2IMEX step, ko.c:444-570
+// body of one RK2IMEX step, ko.c:444-570, at the clock's (t, dt); the
+// coefficients are the reference's expressions of dt
 template <class Ex>
-void step_rk2imex(Ex &ex, const GridBufs &s, double t, double dt)
+void step_rk2imex(Ex &ex, const GridBufs &s)
 {
   const double gamma = 1. - 1. / sqrt(2.);
-  comb(ex, s, s.ut0, 1., s.u);
-  implicit_op(ex, s, t, dt * gamma);
-  comb(ex, s, s.drt1, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.ut0);
-  comb(ex, s, s.ut1, 1., s.u);
+  const Coef one{Coef::CONST, 1.}, idtg{Coef::C_OVER_DTG, 1.}, midtg{Coef::C_OVER_DTG, -1.};
+  const Coef idt{Coef::C_OVER_DT, 1.}, midt{Coef::C_OVER_DT, -1.}, dt1{Coef::DT_TIMES_C, 1.};
+  const Coef dtc{Coef::DT_TIMES_C, 1. - 2. * gamma}, dth{Coef::DT_OVER_C, 2.};
+  comb(ex, s, gamma, s.ut0, one, s.u);
+  implicit_op(ex, s, gamma);
+  comb(ex, s, gamma, s.drt1, idtg, s.u, midtg, s.ut0);
+  comb(ex, s, gamma, s.ut1, one, s.u);
   u2p(ex, s);
   do_correct(ex, s);
-  explicit_op(ex, s, dt);
-  comb(ex, s, s.dut1, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
-  comb(ex, s, s.u, 1., s.ut0, (dt), s.dut1, (dt * (1. - 2. * gamma)), s.drt1);
-  comb(ex, s, s.uforget, 1., s.u);
+  explicit_op(ex, s);
+  comb(ex, s, gamma, s.dut1, idt, s.u, midt, s.ut1);
+  comb(ex, s, gamma, s.u, one, s.ut0, dt1, s.dut1, dtc, s.drt1);
+  comb(ex, s, gamma, s.uforget, one, s.u);
   u2p(ex, s);
   do_correct(ex, s);
-  implicit_op(ex, s, t, gamma * dt);
-  comb(ex, s, s.drt2, (1. / (dt * gamma)), s.u, (-1. / (dt * gamma)), s.uforget);
-  comb(ex, s, s.ut1, 1., s.u);  // ut2
+  implicit_op(ex, s, gamma);
+  comb(ex, s, gamma, s.drt2, idtg, s.u, midtg, s.uforget);
+  comb(ex, s, gamma, s.ut1, one, s.u);  // ut2
   u2p(ex, s);
   do_correct(ex, s);
-  explicit_op(ex, s, dt);
-  comb(ex, s, s.dut2, (1. / (dt)), s.u, (-1. / (dt)), s.ut1);
-  comb(ex, s, s.u, 1., s.ut0, (dt / 2.), s.dut1, (dt / 2.), s.dut2);
-  comb(ex, s, s.u, 1., s.u, (dt / 2.), s.drt1, (dt / 2.), s.drt2);
+  explicit_op(ex, s);
+  comb(ex, s, gamma, s.dut2, idt, s.u, midt, s.ut1);
+  comb(ex, s, gamma, s.u, one, s.ut0, dth, s.dut1, dth, s.dut2);
+  comb(ex, s, gamma, s.u, one, s.u, dth, s.drt1, dth, s.drt2);
   u2p(ex, s);
   do_correct(ex, s);
 }

@@ -28,6 +28,7 @@
 // chain would do without it.  A cell solved by that pass reports attempt 6.
 //
 // Src provides, for a cell index c:
+//   time()                         global time of the solve (device side)
 //   load(c, uu00, pp00, g, dt)     inputs of the solve
 //   init(c)                        before the first attempt
 //   add_iters(c, it)               after every attempt
@@ -128,9 +129,10 @@ __global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Sr
 // attempt a on the cells of one list, all iterating on primitives WP with
 // equation WE in frame WF
 template <int MAG, int RAD, int WP, int WE, int WF, bool LEARN, class Src>
-__global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, double global_time,
-                                                  int *counts, const double *gxg, const double *gxr, long n_all)
+__global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, int *counts,
+                                                  const double *gxg, const double *gxr, long n_all)
 {
+  const double global_time = src.time();
   constexpr int NV = Lay<MAG, RAD>::NV;
   const int lane = threadIdx.x & 63;
   const int kind = (WP == P_MHD) ? 0 : 1;
@@ -255,13 +257,13 @@ int resident_blocks(K kernel, int bs, long n, unsigned *blocks)
 }
 
 template <int MAG, int WP, int WE, int WF, bool LEARN = false, class Src>
-int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, double gt, int *counts,
+int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, int *counts,
                     hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr)
 {
   unsigned blocks = 0;
   int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src>, 256, n, &blocks);
   if (rc) return rc;
-  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, gt, counts, gxg, gxr, n);
+  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, counts, gxg, gxr, n);
   return 0;
 }
 
@@ -271,8 +273,7 @@ int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const 
 // after learned_maxiter iterations when that is positive and below
 // RADIMPMAXITER.
 template <int MAG, class Src>
-int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, double gt, int *counts,
-                    hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr, int learned_maxiter = 0)
+int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs &b, int *counts, hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr, int learned_maxiter = 0)
 {
   const unsigned nb = (unsigned)((n + 255) / 256);
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
@@ -281,21 +282,21 @@ int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs
     k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxg, gxr);
     koral_params Pl = P;
     if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
-    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gxg, gxr);
-    if (!rc && gxr) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, gt, counts, s, gxg, gxr);
+    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, counts, s, gxg, gxr);
+    if (!rc && gxr) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, counts, s, gxg, gxr);
   } else {
     k_imp_classify<MAG, 1, Src><<<nb, 256, 0, s>>>(P, src, n, b);
   }
   for (int a = 0; a < 6 && !rc; a++) {
     if (a >= 4) {
-      rc = imp_launch_pass<MAG, P_MHD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);
-      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENTROPYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+      rc = imp_launch_pass<MAG, P_MHD, ENTROPYEQ, LAB>(P, src, n, a, b, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENTROPYEQ, LAB>(P, src, n, a, b, counts, s);
     } else if (a == 1 || a == 3) {
-      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, FF>(P, src, n, a, b, gt, counts, s);
-      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, FF>(P, src, n, a, b, gt, counts, s);
+      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, FF>(P, src, n, a, b, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, FF>(P, src, n, a, b, counts, s);
     } else {
-      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB>(P, src, n, a, b, gt, counts, s);
-      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB>(P, src, n, a, b, gt, counts, s);
+      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB>(P, src, n, a, b, counts, s);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB>(P, src, n, a, b, counts, s);
     }
   }
   return rc;

@@ -142,6 +142,8 @@ struct BatchCells {
   double *uu, *pp;
   const double *g20, *G20, *dt;
   int *flags, *info;
+  double gt;
+  __device__ __forceinline__ double time() const { return gt; }
   __device__ __forceinline__ void load(long c, double *uu00, double *pp00, Geom &g, double &dtc) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
@@ -351,11 +353,11 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   int rc = acquire_set(n, s, &set);
   if (rc) return rc;
   if (P->mag) {
-    BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-    rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s);
+    BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<1>(*P, src, n, set->b, nullptr, s);
   } else {
-    BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-    rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s);
+    BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<0>(*P, src, n, set->b, nullptr, s);
   }
   int rc2 = release_set(set, s);
   if (rc) return rc;
@@ -388,11 +390,11 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, 
   const double *gxr = rad ? set->gxr : nullptr;
   if (!rc) {
     if (P->mag) {
-      BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<1>(*P, src, n, set->b, global_time, nullptr, s, set->gxg, gxr, maxiter);
+      BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+      rc = implicit_passes<1>(*P, src, n, set->b, nullptr, s, set->gxg, gxr, maxiter);
     } else {
-      BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info};
-      rc = implicit_passes<0>(*P, src, n, set->b, global_time, nullptr, s, set->gxg, gxr, maxiter);
+      BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+      rc = implicit_passes<0>(*P, src, n, set->b, nullptr, s, set->gxg, gxr, maxiter);
     }
   }
   int rc2 = release_set(set, s);

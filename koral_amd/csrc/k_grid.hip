@@ -111,7 +111,9 @@ struct GridCells {
   DGrid G;
   double *u, *p;
   signed char *fl;
-  double dt;
+  const double *clk;  // device clock {t, dt}
+  double fac;         // the solve's time step is dt * fac
+  __device__ __forceinline__ double time() const { return clk[0]; }
   __device__ __forceinline__ long cell(long i, int &ix, int &iy) const
   {
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
@@ -126,7 +128,7 @@ struct GridCells {
     geom_cell(G, ix, iy, g);
     load_p<MAG, 1>(p, G.ncell, c, pp00);
     load_p<MAG, 1>(u, G.ncell, c, uu00);
-    dtc = dt;
+    dtc = clk[1] * fac;
   }
   __device__ __forceinline__ void init(long) const {}
   __device__ __forceinline__ void add_iters(long, int) const {}
@@ -193,8 +195,8 @@ struct HipExec {
   void implicit_cells(const OpImplicit<MAG> &op, int *out)
   {
     timed(op, [&](long n) {
-      GridCells<MAG> src{op.G, op.u, op.p, op.fl, op.dt};
-      int r = implicit_passes<MAG>(op.P, src, n, imp, op.t, out, s);
+      GridCells<MAG> src{op.G, op.u, op.p, op.fl, op.clk, op.fac};
+      int r = implicit_passes<MAG>(op.P, src, n, imp, out, s);
       if (r && !rc) rc = r;
     });
   }
@@ -292,9 +294,36 @@ int refuse_mag_rad(const koral_hip_grid *g, const char *what)
   return 0;
 }
 
+// device clock and scalars, set and advanced by kernels so that a sequence
+// of steps needs no host round trip
+__global__ void k_clock_set(double *clock, double t, double dt)
+{
+  clock[0] = t;
+  clock[1] = dt;
+}
+__global__ void k_zero_scalars(double *tsd, int *counters, int ncounters)
+{
+  if (threadIdx.x == 0) tsd[0] = 0.;
+  if ((int)threadIdx.x < ncounters) counters[threadIdx.x] = 0;
+}
+// after a step: t += dt, dt = 1 / tstepdenmax (ko.c:399), tstepdenmax = 0
+__global__ void k_clock_advance(double *clock, double *tsd)
+{
+  clock[0] = clock[0] + clock[1];
+  clock[1] = 1. / tsd[0];
+  tsd[0] = 0.;
+}
+
+int set_clock(koral_hip_grid *g, double t, double dt)
+{
+  k_clock_set<<<1, 1, 0, g->s>>>(g->b.clock, t, dt);
+  return check_launch("clock");
+}
+
 int zero_tsd(koral_hip_grid *g)
 {
-  return check_hip(hipMemsetAsync(g->b.tsd, 0, sizeof(double), g->s), "memset tsd");
+  k_zero_scalars<<<1, 64, 0, g->s>>>(g->b.tsd, g->b.counters, 0);
+  return check_launch("zero tsd");
 }
 
 int read_tsd(koral_hip_grid *g, double *tsd)
@@ -360,6 +389,7 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   double **work[7] = {&s.ut0, &s.ut1, &s.uforget, &s.dut1, &s.dut2, &s.drt1, &s.drt2};
   for (int k = 0; k < 7 && !rc; k++) rc = g->alloc(work[k], nv * nc);
   if (!rc) rc = g->alloc(&s.tsd, 1);
+  if (!rc) rc = g->alloc(&s.clock, 4);
   if (!rc) rc = g->alloc(&s.counters, 8);
   const long nd = (long)G.nx * G.ny * G.nz;
   for (int i = 0; i < 2 && !rc; i++)
@@ -532,13 +562,13 @@ int koral_hip_tstepden(koral_hip_grid *g, double *tstepdenmax)
 
 int koral_hip_op_explicit(koral_hip_grid *g, double t, double dt, double *tstepdenmax)
 {
-  (void)t;
   if (!g) return set_error(KORAL_E_ARG, "null grid");
   int rc = refuse_mag_rad(g, "op_explicit");
   if (!rc) rc = zero_tsd(g);
+  if (!rc) rc = set_clock(g, t, dt);
   if (rc) return rc;
   HipExec ex = g->ex();
-  explicit_op(ex, g->b, dt);
+  explicit_op(ex, g->b);
   rc = check_launch("op_explicit");
   if (!rc && tstepdenmax) rc = read_tsd(g, tstepdenmax);
   return rc;
@@ -563,8 +593,10 @@ int koral_hip_do_correct(koral_hip_grid *g)
 int koral_hip_op_implicit(koral_hip_grid *g, double t, double dt)
 {
   if (!g) return set_error(KORAL_E_ARG, "null grid");
+  int rc = set_clock(g, t, dt);
+  if (rc) return rc;
   HipExec ex = g->ex();
-  implicit_op(ex, g->b, t, dt);
+  implicit_op(ex, g->b, 1.);
   if (ex.rc) return ex.rc;
   return check_launch("op_implicit");
 }
@@ -573,15 +605,40 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
 {
   if (!g) return set_error(KORAL_E_ARG, "null grid");
   int rc = refuse_mag_rad(g, "step_rk2imex");
-  if (!rc) rc = zero_tsd(g);
-  if (!rc) rc = check_hip(hipMemsetAsync(g->b.counters, 0, 8 * sizeof(int), g->s), "memset counters");
+  if (rc) return rc;
+  k_zero_scalars<<<1, 64, 0, g->s>>>(g->b.tsd, g->b.counters, 8);
+  rc = set_clock(g, t, dt);
   if (rc) return rc;
   HipExec ex = g->ex();
-  step_rk2imex(ex, g->b, t, dt);
+  step_rk2imex(ex, g->b);
   rc = ex.rc ? ex.rc : check_launch("step_rk2imex");
   if (!rc && tstepdenmax) rc = check_hip(hipMemcpyAsync(tstepdenmax, g->b.tsd, sizeof(double), hipMemcpyDeviceToHost, g->s), "tsd");
   if (!rc && counters) rc = check_hip(hipMemcpyAsync(counters, g->b.counters, 6 * sizeof(int), hipMemcpyDeviceToHost, g->s), "counters");
   if (!rc && (tstepdenmax || counters)) rc = check_hip(hipStreamSynchronize(g->s), "sync");
+  return rc;
+}
+
+int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, int *counters)
+{
+  if (!g || !t || !dt || nsteps < 0) return set_error(KORAL_E_ARG, "run_rk2imex: bad arguments");
+  int rc = refuse_mag_rad(g, "run_rk2imex");
+  if (rc) return rc;
+  k_zero_scalars<<<1, 64, 0, g->s>>>(g->b.tsd, g->b.counters, 8);
+  rc = set_clock(g, *t, *dt);
+  HipExec ex = g->ex();
+  for (int k = 0; k < nsteps && !rc; k++) {
+    step_rk2imex(ex, g->b);
+    k_clock_advance<<<1, 1, 0, g->s>>>(g->b.clock, g->b.tsd);
+    rc = ex.rc ? ex.rc : check_launch("run_rk2imex");
+  }
+  double clk[2];
+  if (!rc) rc = check_hip(hipMemcpyAsync(clk, g->b.clock, 2 * sizeof(double), hipMemcpyDeviceToHost, g->s), "clock");
+  if (!rc && counters) rc = check_hip(hipMemcpyAsync(counters, g->b.counters, 6 * sizeof(int), hipMemcpyDeviceToHost, g->s), "counters");
+  if (!rc) rc = check_hip(hipStreamSynchronize(g->s), "sync");
+  if (!rc) {
+    *t = clk[0];
+    *dt = clk[1];
+  }
   return rc;
 }
 

@@ -44,6 +44,7 @@ def _sig(L):
         "koral_hip_do_correct": [_vp],
         "koral_hip_op_implicit": [_vp, d, d],
         "koral_hip_step_rk2imex": [_vp, d, d, _vp, _vp],
+        "koral_hip_run_rk2imex": [_vp, i, _vp, _vp, _vp],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -54,7 +55,7 @@ def _sig(L):
     return L
 
 
-NKINDS = 14
+NKINDS = 32  # at least the library's operator kinds; unnamed kinds are skipped
 
 
 class DeviceGrid:
@@ -147,7 +148,7 @@ class DeviceGrid:
         check(self.L.koral_hip_grid_timing(self.h, int(enable), ms.ctypes.data_as(_vp), units.ctypes.data_as(_vp),
                                            nl.ctypes.data_as(_vp), NKINDS), "grid_timing")
         return {self.L.koral_hip_grid_op_name(k).decode(): (float(ms[k]), int(units[k]), int(nl[k]))
-                for k in range(NKINDS) if nl[k]}
+                for k in range(NKINDS) if nl[k] and self.L.koral_hip_grid_op_name(k)}
 
     def p2u(self):
         check(self.L.koral_hip_grid_p2u(self.h), "grid_p2u")
@@ -180,6 +181,19 @@ class DeviceGrid:
         self.p2u()
         self.set_bc()
         self.dt = 1. / self.tstepden()
+
+    def run(self, nsteps):
+        """nsteps steps with the time kept on the device (dt of each step
+        from the previous step's time-step density, as step() does); one
+        synchronisation at the end.  self.counters sums the steps."""
+        t = ctypes.c_double(self.t)
+        dt = ctypes.c_double(self.dt)
+        c = np.zeros(6, np.int32)
+        check(self.L.koral_hip_run_rk2imex(self.h, int(nsteps), ctypes.byref(t), ctypes.byref(dt),
+                                           c.ctypes.data_as(_vp)), "run_rk2imex")
+        self.t, self.dt = t.value, dt.value
+        self.counters = c
+        return self.dt
 
     def step(self, sync=True):
         """One RK2IMEX step with dt from the previous step (ko.c:399)."""

@@ -173,6 +173,30 @@ def test_grid_step_parity(oracle, kind, nsteps):
         dg.dt = dt
 
 
+@pytest.mark.parametrize("kind", ["tube", "torus_mhd_polar"])
+def test_grid_run_matches_steps(kind):
+    """Steps with the time kept on the device (koral_hip_run_rk2imex) give
+    the same state, time, step and counters as steps driven from the host."""
+    from koral_amd.gridrun import DeviceGrid
+    P, spec, p = _setup(kind)
+    _say("device-clock run vs host steps", kind)
+    a, b = DeviceGrid(P, spec), DeviceGrid(P, spec)
+    for dg in (a, b):
+        dg.upload(p)
+        dg.init()
+    tot = np.zeros(6, np.int64)
+    for _ in range(3):
+        a.step()
+        tot += a.counters
+    b.run(3)
+    pa, ua, fa = a.download()
+    pb, ub, fb = b.download()
+    _say("  t %.17g vs %.17g, dt %.17g vs %.17g" % (a.t, b.t, a.dt, b.dt))
+    assert a.t == b.t and a.dt == b.dt
+    assert np.array_equal(pa, pb) and np.array_equal(ua, ub) and np.array_equal(fa, fb)
+    assert np.array_equal(tot, np.asarray(b.counters, np.int64))
+
+
 @pytest.mark.parametrize("kind", ["torus_mhd", "torus_rad"])
 def test_grid_polar_axis(oracle, kind):
     """do_correct with the polar-axis correction (NCCORRECTPOLAR = 2)."""
