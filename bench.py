@@ -320,7 +320,12 @@ def bench_grid(args, kind):
         dg.step()
     zones = n[0] * n[1] * n[2]
     # the steps run with the time kept on the device (no host round trip
-    # between steps; dt of each step from the previous one, as ko.c does)
+    # between steps; dt of each step from the previous one, as ko.c does);
+    # --graph replays one captured HIP graph per step instead (measured a
+    # little slower than the stream launches on MI355X)
+    if args.graph:
+        dg.use_graph(True)
+        dg.run(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dg.run(args.steps)
@@ -407,6 +412,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the single-GPU build holds the whole domain "

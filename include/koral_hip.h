@@ -237,6 +237,10 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
    reference's loop does, with no host round trip between steps.  On return *t, *dt are the
    time and step of the next step; counters[6] (may be NULL) sum the step counters. */
 int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, int *counters);
+/* With enable != 0 (a stream must be set), koral_hip_run_rk2imex captures one step into a
+   HIP graph on first use and replays it; enable = 0 drops the graph.  Per-operator timing,
+   when on, runs the steps without the graph. */
+int koral_hip_grid_use_graph(koral_hip_grid *g, int enable);
 
 #ifdef __cplusplus
 }

@@ -247,6 +247,18 @@ struct koral_hip_grid {
   Timing timing;
   std::vector<void *> allocs;
   double *stage = nullptr;  // AoS staging for upload/download
+  // one RK2IMEX step plus the clock advance, captured once and replayed by
+  // koral_hip_run_rk2imex when graphs are enabled
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  void drop_graph()
+  {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
 
   template <class T>
   int alloc(T **ptr, size_t count)
@@ -261,6 +273,7 @@ struct koral_hip_grid {
   }
   ~koral_hip_grid()
   {
+    drop_graph();
     for (void *q : allocs) (void)hipFree(q);
   }
   HipExec ex()
@@ -618,6 +631,15 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
   return rc;
 }
 
+int koral_hip_grid_use_graph(koral_hip_grid *g, int enable)
+{
+  if (!g) return set_error(KORAL_E_ARG, "null grid");
+  if (enable && !g->s) return set_error(KORAL_E_ARG, "grid_use_graph: set a stream first (graphs are captured on it)");
+  g->use_graph = enable != 0;
+  if (!g->use_graph) g->drop_graph();
+  return 0;
+}
+
 int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, int *counters)
 {
   if (!g || !t || !dt || nsteps < 0) return set_error(KORAL_E_ARG, "run_rk2imex: bad arguments");
@@ -625,11 +647,32 @@ int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, 
   if (rc) return rc;
   k_zero_scalars<<<1, 64, 0, g->s>>>(g->b.tsd, g->b.counters, 8);
   rc = set_clock(g, *t, *dt);
-  HipExec ex = g->ex();
-  for (int k = 0; k < nsteps && !rc; k++) {
-    step_rk2imex(ex, g->b);
-    k_clock_advance<<<1, 1, 0, g->s>>>(g->b.clock, g->b.tsd);
-    rc = ex.rc ? ex.rc : check_launch("run_rk2imex");
+  const bool graph = g->use_graph && g->s && !g->timing.on;
+  if (!rc && graph && !g->gexec) {
+    // capture one step; every value that changes between steps lives in
+    // device memory (clock, counters, work lists), so the graph is replayed
+    // unchanged
+    rc = check_hip(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (!rc) {
+      HipExec ex = g->ex();
+      step_rk2imex(ex, g->b);
+      k_clock_advance<<<1, 1, 0, g->s>>>(g->b.clock, g->b.tsd);
+      int rc1 = ex.rc ? ex.rc : check_launch("capture step");
+      int rc2 = check_hip(hipStreamEndCapture(g->s, &g->graph), "end capture");
+      rc = rc1 ? rc1 : rc2;
+    }
+    if (!rc) rc = check_hip(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0), "graph instantiate");
+    if (rc) g->drop_graph();
+  }
+  if (graph) {
+    for (int k = 0; k < nsteps && !rc; k++) rc = check_hip(hipGraphLaunch(g->gexec, g->s), "graph launch");
+  } else {
+    HipExec ex = g->ex();
+    for (int k = 0; k < nsteps && !rc; k++) {
+      step_rk2imex(ex, g->b);
+      k_clock_advance<<<1, 1, 0, g->s>>>(g->b.clock, g->b.tsd);
+      rc = ex.rc ? ex.rc : check_launch("run_rk2imex");
+    }
   }
   double clk[2];
   if (!rc) rc = check_hip(hipMemcpyAsync(clk, g->b.clock, 2 * sizeof(double), hipMemcpyDeviceToHost, g->s), "clock");

@@ -45,6 +45,7 @@ def _sig(L):
         "koral_hip_op_implicit": [_vp, d, d],
         "koral_hip_step_rk2imex": [_vp, d, d, _vp, _vp],
         "koral_hip_run_rk2imex": [_vp, i, _vp, _vp, _vp],
+        "koral_hip_grid_use_graph": [_vp, i],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -181,6 +182,10 @@ class DeviceGrid:
         self.p2u()
         self.set_bc()
         self.dt = 1. / self.tstepden()
+
+    def use_graph(self, enable=True):
+        """Replay one captured step per step in run() (needs a stream)."""
+        check(self.L.koral_hip_grid_use_graph(self.h, int(enable)), "grid_use_graph")
 
     def run(self, nsteps):
         """nsteps steps with the time kept on the device (dt of each step
