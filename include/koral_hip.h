@@ -237,6 +237,19 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
    reference's loop does, with no host round trip between steps.  On return *t, *dt are the
    time and step of the next step; counters[6] (may be NULL) sum the step counters. */
 int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, int *counters);
+/* fprint_restartfile_bin (fileop.c:911-982): folder/res%04d.head ("## nout1 nout2 t PROBLEM
+   NX NY NZ", t as %10.6e) and folder/res%04d.dat (NX*NY*NZ int index triples, ix outer and iz
+   inner, then NV doubles of primitives per cell in the same order) from the device state;
+   folder/reslast.{head,dat} are linked to them. */
+int koral_hip_grid_write_restart(koral_hip_grid *g, const char *folder, int nout1, int nout2, double t,
+                                 int problem);
+/* fread_restartfile_bin (fileop.c:1009-1108): reads res%04d (nout1 >= 0) or reslast
+   (nout1 < 0) into the interior primitives and sets u = p2u(p) there; ghosts are left for
+   set_bc.  Returns 1 without touching the grid when the header file does not exist (the
+   reference then starts from scratch).  *t from the header; *nfout1 = file number + 1 and
+   *nfout2 (may be NULL) as the reference sets nfout1, nfout2. */
+int koral_hip_grid_read_restart(koral_hip_grid *g, const char *folder, int nout1, double *t, int *nfout1,
+                                int *nfout2);
 /* With enable != 0 (a stream must be set), koral_hip_run_rk2imex captures one step into a
    HIP graph on first use and replays it; enable = 0 drops the graph.  Per-operator timing,
    when on, runs the steps without the graph. */

@@ -22,7 +22,7 @@ struct DGrid {
   int bc[6];
 };
 
-KD_INL long cidx(const DGrid &G, int ix, int iy, int iz)
+KD_HD long cidx(const DGrid &G, int ix, int iy, int iz)
 {
   long y = G.ny > 1 ? iy + G.ngy : 0;
   long z = G.nz > 1 ? iz + G.ngz : 0;

@@ -860,10 +860,11 @@ inline Box wavespeed_box(const koral_params &P, const DGrid &G)
 }
 
 template <class Ex>
-void grid_p2u(Ex &ex, const GridBufs &s)
+void grid_p2u(Ex &ex, const GridBufs &s, bool interior_only = false)
 {
   const DGrid &G = s.G;
   Box b{-G.ngx, -G.ngy, -G.ngz, G.sx, G.sy, G.sz};
+  if (interior_only) b = Box{0, 0, 0, G.nx, G.ny, G.nz};
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
     ex.each(OpP2U<T::MAG, T::RAD>{s.P, G, s.p, s.u, b});

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
+#include <unistd.h>
 #include <vector>
 #include "koral_hip.h"
 #include "grid_ops.hpp"
@@ -629,6 +630,105 @@ int koral_hip_step_rk2imex(koral_hip_grid *g, double t, double dt, double *tstep
   if (!rc && counters) rc = check_hip(hipMemcpyAsync(counters, g->b.counters, 6 * sizeof(int), hipMemcpyDeviceToHost, g->s), "counters");
   if (!rc && (tstepdenmax || counters)) rc = check_hip(hipStreamSynchronize(g->s), "sync");
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// restart files (fileop.c:911-982 writer, fileop.c:1009-1108 reader): a text
+// header "## nout1 nout2 t PROBLEM NX NY NZ" and a binary body of NX*NY*NZ
+// int index triples (ix outer, iz inner) followed by NV doubles of primitives
+// per cell in the same order
+// ---------------------------------------------------------------------------
+int koral_hip_grid_write_restart(koral_hip_grid *g, const char *folder, int nout1, int nout2, double t,
+                                 int problem)
+{
+  if (!g || !folder || nout1 < 0) return set_error(KORAL_E_ARG, "write_restart: bad arguments");
+  const DGrid &G = g->b.G;
+  const int nv = g->b.P.nv;
+  std::vector<double> p((size_t)nv * G.ncell);
+  int rc = koral_hip_grid_download(g, p.data(), nullptr, nullptr);
+  if (rc) return rc;
+  char name[1024], link[1024], target[64];
+  snprintf(name, sizeof name, "%s/res%04d.head", folder, nout1);
+  FILE *f = fopen(name, "w");
+  if (!f) return set_error(KORAL_E_STATE, "write_restart: cannot create the header file");
+  int bad = fprintf(f, "## %5d %5d %10.6e %5d %5d %5d %5d\n", nout1, nout2, t, problem, G.nx, G.ny, G.nz) < 0;
+  bad |= fclose(f) != 0;
+  snprintf(name, sizeof name, "%s/res%04d.dat", folder, nout1);
+  f = bad ? nullptr : fopen(name, "wb");
+  if (!f) return set_error(KORAL_E_STATE, "write_restart: cannot write the restart files");
+  for (int ix = 0; ix < G.nx && !bad; ix++)
+    for (int iy = 0; iy < G.ny && !bad; iy++)
+      for (int iz = 0; iz < G.nz && !bad; iz++) {
+        const int idx[3] = {ix, iy, iz};
+        bad |= fwrite(idx, sizeof(int), 3, f) != 3;
+      }
+  for (int ix = 0; ix < G.nx && !bad; ix++)
+    for (int iy = 0; iy < G.ny && !bad; iy++)
+      for (int iz = 0; iz < G.nz && !bad; iz++)
+        bad |= fwrite(&p[(size_t)nv * cidx(G, ix, iy, iz)], sizeof(double), nv, f) != (size_t)nv;
+  bad |= fclose(f) != 0;
+  if (bad) return set_error(KORAL_E_STATE, "write_restart: write error");
+  // reslast.{head,dat} point at the newest files, as the reference's links do
+  const char *ext[2] = {"head", "dat"};
+  for (int k = 0; k < 2; k++) {
+    snprintf(link, sizeof link, "%s/reslast.%s", folder, ext[k]);
+    snprintf(target, sizeof target, "res%04d.%s", nout1, ext[k]);
+    (void)unlink(link);
+    if (symlink(target, link) != 0) return set_error(KORAL_E_STATE, "write_restart: cannot link reslast");
+  }
+  return 0;
+}
+
+int koral_hip_grid_read_restart(koral_hip_grid *g, const char *folder, int nout1, double *t, int *nfout1,
+                                int *nfout2)
+{
+  if (!g || !folder || !t) return set_error(KORAL_E_ARG, "read_restart: bad arguments");
+  const DGrid &G = g->b.G;
+  const int nv = g->b.P.nv;
+  char name[1024];
+  if (nout1 >= 0)
+    snprintf(name, sizeof name, "%s/res%04d.head", folder, nout1);
+  else
+    snprintf(name, sizeof name, "%s/reslast.head", folder);
+  FILE *f = fopen(name, "r");
+  if (!f) return 1;  // no restart file: start from scratch
+  int ip[6];
+  double tt = 0.;
+  const int nr = fscanf(f, "## %d %d %lf %d %d %d %d\n", &ip[0], &ip[1], &tt, &ip[2], &ip[3], &ip[4], &ip[5]);
+  fclose(f);
+  if (nr != 7) return set_error(KORAL_E_STATE, "read_restart: malformed header");
+  if (ip[3] != G.nx || ip[4] != G.ny || ip[5] != G.nz)
+    return set_error(KORAL_E_STATE, "read_restart: grid size of the file differs from the grid");
+  if (nout1 >= 0)
+    snprintf(name, sizeof name, "%s/res%04d.dat", folder, nout1);
+  else
+    snprintf(name, sizeof name, "%s/reslast.dat", folder);
+  f = fopen(name, "rb");
+  if (!f) return set_error(KORAL_E_STATE, "read_restart: cannot open the body file");
+  const long ncells = (long)G.nx * G.ny * G.nz;
+  std::vector<int> idx((size_t)3 * ncells);
+  int bad = fread(idx.data(), sizeof(int), idx.size(), f) != idx.size();
+  for (long c = 0; c < ncells && !bad; c++)
+    bad |= idx[3 * c] < 0 || idx[3 * c] >= G.nx || idx[3 * c + 1] < 0 || idx[3 * c + 1] >= G.ny ||
+           idx[3 * c + 2] < 0 || idx[3 * c + 2] >= G.nz;
+  std::vector<double> p((size_t)nv * G.ncell);
+  if (!bad) bad = koral_hip_grid_download(g, p.data(), nullptr, nullptr) != 0;  // ghosts kept
+  for (long c = 0; c < ncells && !bad; c++) {
+    double *dst = &p[(size_t)nv * cidx(G, idx[3 * c], idx[3 * c + 1], idx[3 * c + 2])];
+    bad |= fread(dst, sizeof(double), nv, f) != (size_t)nv;
+  }
+  fclose(f);
+  if (bad) return set_error(KORAL_E_STATE, "read_restart: truncated or inconsistent body file");
+  int rc = koral_hip_grid_upload(g, p.data(), nullptr, nullptr);
+  if (rc) return rc;
+  HipExec ex = g->ex();
+  grid_p2u(ex, g->b, true);
+  rc = check_hip(hipStreamSynchronize(g->s), "read_restart sync");
+  if (rc) return rc;
+  *t = tt;
+  if (nfout1) *nfout1 = ip[0] + 1;
+  if (nfout2) *nfout2 = ip[1];
+  return check_launch("read_restart");
 }
 
 int koral_hip_grid_use_graph(koral_hip_grid *g, int enable)

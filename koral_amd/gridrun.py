@@ -46,6 +46,8 @@ def _sig(L):
         "koral_hip_step_rk2imex": [_vp, d, d, _vp, _vp],
         "koral_hip_run_rk2imex": [_vp, i, _vp, _vp, _vp],
         "koral_hip_grid_use_graph": [_vp, i],
+        "koral_hip_grid_write_restart": [_vp, ctypes.c_char_p, i, i, d, i],
+        "koral_hip_grid_read_restart": [_vp, ctypes.c_char_p, i, _vp, _vp, _vp],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -182,6 +184,24 @@ class DeviceGrid:
         self.p2u()
         self.set_bc()
         self.dt = 1. / self.tstepden()
+
+    def write_restart(self, folder, nout1, nout2=0, problem=0):
+        """KORAL restart files res%04d.head/.dat of the current state at self.t."""
+        check(self.L.koral_hip_grid_write_restart(self.h, str(folder).encode(), int(nout1), int(nout2),
+                                                  float(self.t), int(problem)), "write_restart")
+
+    def read_restart(self, folder, nout1=-1):
+        """Reads res%04d (or reslast for nout1 < 0); returns (found, next file
+        number) and sets self.t from the header."""
+        t = ctypes.c_double(0.)
+        n1, n2 = ctypes.c_int(0), ctypes.c_int(0)
+        r = self.L.koral_hip_grid_read_restart(self.h, str(folder).encode(), int(nout1), ctypes.byref(t),
+                                               ctypes.byref(n1), ctypes.byref(n2))
+        if r == 1:
+            return False, 0
+        check(r, "read_restart")
+        self.t = t.value
+        return True, n1.value
 
     def use_graph(self, enable=True):
         """Replay one captured step per step in run() (needs a stream)."""

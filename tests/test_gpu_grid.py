@@ -1,6 +1,7 @@
 """GPU parity of the grid operators (ghost fill, explicit operator, u2p sweep
 with fixups, implicit sweep, RK2IMEX step) against the CPU oracle on small
 grids of the three step workloads, through the C-ABI."""
+import os
 import sys
 
 import numpy as np
@@ -260,3 +261,69 @@ def test_field_radiation_explicit_refused():
     with pytest.raises(KoralError, match="disabled"):
         dg.dt = 1e-3
         dg.step()
+
+
+def _read_restart_files(folder, n, nv, shape):
+    """Independent reader of KORAL's restart format (fileop.c:911-982)."""
+    import re
+    head = open(os.path.join(folder, "res%04d.head" % n)).read()
+    m = re.match(r"## +(-?\d+) +(-?\d+) +(\S+) +(-?\d+) +(\d+) +(\d+) +(\d+)\n$", head)
+    assert m, head
+    nx, ny, nz = shape
+    raw = open(os.path.join(folder, "res%04d.dat" % n), "rb").read()
+    assert len(raw) == nx * ny * nz * (3 * 4 + nv * 8)
+    idx = np.frombuffer(raw, np.int32, 3 * nx * ny * nz).reshape(-1, 3)
+    prim = np.frombuffer(raw, np.float64, offset=12 * nx * ny * nz).reshape(-1, nv)
+    return m, head, idx, prim
+
+
+def test_grid_restart_files(oracle, tmp_path):
+    """Restart files written from the device follow KORAL's format, read
+    back bit-for-bit, and a file written independently (cells in shuffled
+    order) is read into the right cells with u = p2u(p)."""
+    from koral_amd.gridrun import DeviceGrid
+    P, spec, p = _setup("torus_mhd")
+    _say("restart files torus_mhd")
+    a = DeviceGrid(P, spec)
+    a.upload(p)
+    a.init()
+    a.step()
+    a.write_restart(tmp_path, 3, nout2=1, problem=7)
+    nx, ny, nz = spec.nx, spec.ny, spec.nz
+    m, head, idx, prim = _read_restart_files(tmp_path, 3, P.nv, (nx, ny, nz))
+    assert [int(m.group(k)) for k in (1, 2, 4, 5, 6, 7)] == [3, 1, 7, nx, ny, nz]
+    assert head == "## %5d %5d %10.6e %5d %5d %5d %5d\n" % (3, 1, a.t, 7, nx, ny, nz)
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    assert np.array_equal(idx, np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1))
+    pa, ua, _ = a.download()
+    inner = pa[spec.interior()]                       # [z, y, x, nv]
+    assert np.array_equal(prim, inner.transpose(2, 1, 0, 3).reshape(-1, P.nv))
+    assert os.readlink(os.path.join(tmp_path, "reslast.dat")) == "res0003.dat"
+    # read back (reslast) into a fresh grid
+    b = DeviceGrid(P, spec)
+    b.upload(p)
+    found, nxt = b.read_restart(tmp_path)
+    assert found and nxt == 4 and b.t == float("%10.6e" % a.t)
+    pb, ub, _ = b.download()
+    assert np.array_equal(pb[spec.interior()], inner)
+    gc, Gc = spec.cell_tables()
+    sl = spec.interior()
+    u_ref = oracle.p2u(P, inner.reshape(-1, P.nv), np.ascontiguousarray(gc[sl]).reshape(-1, 20),
+                       np.ascontiguousarray(Gc[sl]).reshape(-1, 20))
+    assert np.abs(ub[sl].reshape(-1, P.nv) - u_ref).max() <= 1e-14 * np.abs(u_ref).max()
+    # a file in KORAL's format written here, cells in shuffled order
+    rng = np.random.default_rng(5)
+    order = rng.permutation(nx * ny * nz)
+    pnew = inner.transpose(2, 1, 0, 3).reshape(-1, P.nv) * (1. + 1e-3 * rng.random((nx * ny * nz, 1)))
+    d = tmp_path / "other"
+    d.mkdir()
+    (d / "res0000.head").write_text("## %5d %5d %10.6e %5d %5d %5d %5d\n" % (0, 0, 2.5, 7, nx, ny, nz))
+    body = np.ascontiguousarray(np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1)[order], np.int32).tobytes()
+    body += np.ascontiguousarray(pnew[order]).tobytes()
+    (d / "res0000.dat").write_bytes(body)
+    found, nxt = b.read_restart(d, 0)
+    assert found and nxt == 1 and b.t == 2.5
+    pb, _, _ = b.download()
+    assert np.array_equal(pb[sl].transpose(2, 1, 0, 3).reshape(-1, P.nv), pnew)
+    # no file: the reference starts from scratch
+    assert b.read_restart(tmp_path / "none", 9) == (False, 0)
