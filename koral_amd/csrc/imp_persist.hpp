@@ -151,6 +151,14 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
   int iter = 0;
   int n_ok = 0, n_it = 0, n_bad = 0;
   double pp[NV];
+  // The inputs of a lane's current cell, copied once when the lane takes the
+  // cell: the lanes of a wave hold unrelated cells, so reloading them from
+  // the cell arrays every iteration would touch one cache line per lane and
+  // value.  Column threadIdx.x belongs to this lane alone (no barrier).
+  constexpr int NG = 43;  // Geom: g[4][5], G[4][5], gdet, alpha, gttpert
+  static_assert(sizeof(Geom) == NG * sizeof(double), "Geom layout");
+  __shared__ double keep[2 * NV + NG + 1][256];
+  const int me = threadIdx.x;
   for (;;) {
     // failures of the last iteration go to the next attempt's list
     wave_append(push && next_kind == 0, c_push, out0, &out_len[0]);
@@ -179,8 +187,24 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
     if (active) {
       double uu00[NV], pp00[NV], dt;
       Geom g;
-      src.load(c, uu00, pp00, g, dt);
       if (fresh) {
+        src.load(c, uu00, pp00, g, dt);
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+          keep[i][me] = uu00[i];
+          keep[NV + i][me] = pp00[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 5; j++) {
+            keep[2 * NV + 5 * i + j][me] = g.g[i][j];
+            keep[2 * NV + 20 + 5 * i + j][me] = g.G[i][j];
+          }
+        keep[2 * NV + 40][me] = g.gdet;
+        keep[2 * NV + 41][me] = g.alpha;
+        keep[2 * NV + 42][me] = g.gttpert;
+        keep[2 * NV + NG][me] = dt;
 #pragma unroll
         for (int i = 0; i < NV; i++) pp[i] = pp00[i];
         if (LEARN) {
@@ -195,6 +219,23 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
         }
         iter = 0;
         fresh = false;
+      } else {
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+          uu00[i] = keep[i][me];
+          pp00[i] = keep[NV + i][me];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 5; j++) {
+            g.g[i][j] = keep[2 * NV + 5 * i + j][me];
+            g.G[i][j] = keep[2 * NV + 20 + 5 * i + j][me];
+          }
+        g.gdet = keep[2 * NV + 40][me];
+        g.alpha = keep[2 * NV + 41][me];
+        g.gttpert = keep[2 * NV + 42][me];
+        dt = keep[2 * NV + NG][me];
       }
       const int st = imp_newton_step<MAG, RAD, WP, WE, WF>(P, uu00, pp00, g, dt, pp, global_time, iter);
       if (st != IMP_CONT) {
