@@ -1,13 +1,19 @@
 """Benchmark of the MI355X KORAL hot path.  Prints ONE JSON line.
 
-Default workload: the batched implicit radiation source solve on 2^22 seeded
-cells (plain initial guess, full six-attempt fallback chain), FP64.  A step is
-one solve of the whole batch from the same initial state (restored by a
-device-to-device copy inside the timed region).
+Default workload (--workload grmhd): the 3-D GRMHD RK2IMEX step of KORAL on
+the MKS1 Kerr (a = 0.9) magnetised torus at 128x128x64 (MP5, HLL, flux-CT,
+Noble u2p), FP64, device-resident, in zone-updates/s.  The field + radiation
+step is refused by this build (see the C-ABI's KORAL_E_STATE), so the
+GRRMHD+M1 zone-update rate is not measured.  Other workloads:
+  implicit  -- batched implicit radiation-source solve, 2^22 seeded cells,
+               plain guess (implicit solves/s)
+  learned   -- the same solve with the learned initial guess, trained here
+               on pairs from the plain solver, against the plain solve
+  grrmhd    -- the field + radiation torus step (refused)
 
 The line carries:
-  roofline      -- for the dominant kernel: algorithmic bytes per launch over
-                   its HIP-event-measured duration, against 8 TB/s HBM
+  roofline      -- for the dominant kernel: algorithmic bytes (or FP64
+                   operations) per launch over its HIP-event-measured duration
   cpu_baseline  -- the CPU oracle timed on a bounded sample on the host cores
                    (rank 0, N=1 only)
 """
@@ -23,6 +29,12 @@ sys.path.insert(0, ROOT)
 METRIC = "zone-updates/s (3D GRRMHD+M1, FP64); implicit solves/s learned vs plain"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+
+# HBM bytes per launch of the flux kernel on the 128x128x64 torus, averaged
+# over the three sweep directions: FETCH_SIZE and WRITE_SIZE per dispatch
+# scaled by the same counters on a plain copy of known size (profiles/
+# hbm_traffic_grmhd_torus_128x128x64.md: 452, 696 and 862 MB).
+FLUX_TRAFFIC_128x128x64 = 670e6
 
 # Algorithmic FP64 operations per implicit solve of the seeded batch workload
 # (all six fallback attempts included), counted by the operation-counting
@@ -260,6 +272,9 @@ def bench_learned(args):
                      "infer": {"achieved_TFLOPs": inf_tf, "peak_TFLOPs": 2500.0, "frac": inf_tf / 2500.0,
                                "bound": "mfma (bf16)"}},
     }
+    if not args.no_cpu_baseline:
+        # the reference's CPU path is the plain solve (it has no learned guess)
+        res["cpu_baseline"] = cpu_baseline_implicit(P, pp, g20, G20, dt, args)
     return res
 
 
@@ -361,7 +376,8 @@ def bench_grid(args, kind):
                    "counters_per_step": (counters / args.steps).tolist(),
                    "operator_ms_per_step": {k: v[0] / args.steps for k, v in sorted(tm.items(), key=lambda kv: -kv[1][0])}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": FLUX_TRAFFIC_128x128x64 if (dom == "flux" and n == (128, 128, 64)) else None,
                      "kernel": dom, "kernel_ms": per_launch * 1e3, "launches": nl,
                      "algorithmic_bytes_per_launch": bytes_launch},
     }
@@ -398,9 +414,9 @@ def cpu_baseline_grid(P, kind, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="implicit", choices=["implicit", "learned", "grmhd", "grrmhd"])
+    ap.add_argument("--workload", default="grmhd", choices=["implicit", "learned", "grmhd", "grrmhd"])
     ap.add_argument("--grid", type=int, nargs=3, default=None)
     ap.add_argument("--cpu-grid", type=int, nargs=3, default=[48, 48, 24])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
