@@ -206,6 +206,9 @@ int koral_hip_grid_download_work(koral_hip_grid *g, int which, double *out);
    come from koral_hip_grid_op_name (14 kinds).  No reference counterpart
    (KORAL times whole steps only, ko.c:771). */
 int koral_hip_grid_timing(koral_hip_grid *g, int enable, double *ms, long *units, int *launches, int nkinds);
+/* Restricts the timing events to the operator kinds whose bit (1 << kind) is set in mask
+   (all kinds by default), so one operator can be timed inside a timed sequence of steps. */
+int koral_hip_grid_timing_mask(koral_hip_grid *g, unsigned mask);
 const char *koral_hip_grid_op_name(int kind);
 
 /* u = p2u(p) in every cell (as set_initial_profile leaves the state) */

@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_count(Op op, in
 // operator kind when read back.
 struct Timing {
   bool on = false;
+  unsigned mask = ~0u;  // operator kinds that get events while on
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Rec {
@@ -164,7 +165,7 @@ struct HipExec {
     long n = op.size();
     if (n <= 0) return;
     hipEvent_t a = nullptr, b = nullptr;
-    if (tm && tm->on) {
+    if (tm && tm->on && ((tm->mask >> Op::kind) & 1u)) {
       a = tm->next();
       b = tm->next();
     }
@@ -537,6 +538,13 @@ int koral_hip_grid_timing(koral_hip_grid *g, int enable, double *ms, long *units
   T.used = 0;
   T.on = enable != 0;
   return rc;
+}
+
+int koral_hip_grid_timing_mask(koral_hip_grid *g, unsigned mask)
+{
+  if (!g) return set_error(KORAL_E_ARG, "null grid");
+  g->timing.mask = mask;
+  return 0;
 }
 
 const char *koral_hip_grid_op_name(int kind)

@@ -36,6 +36,7 @@ def _sig(L):
         "koral_hip_grid_device_state": [_vp, _vp, _vp, _vp],
         "koral_hip_grid_download_work": [_vp, i, _vp],
         "koral_hip_grid_timing": [_vp, i, _vp, _vp, _vp, i],
+        "koral_hip_grid_timing_mask": [_vp, ctypes.c_uint],
         "koral_hip_grid_p2u": [_vp],
         "koral_hip_set_bc": [_vp],
         "koral_hip_tstepden": [_vp, _vp],
@@ -137,6 +138,17 @@ class DeviceGrid:
             check(self.L.koral_hip_grid_download_work(self.h, d + 1, f.ctypes.data_as(_vp)), "download_work")
             fl.append(np.ascontiguousarray(np.moveaxis(f, 0, -1)))
         return np.ascontiguousarray(np.moveaxis(ws, 0, -1)), fl
+
+    def timing_only(self, names=None):
+        """Time only the named operators (all when names is None)."""
+        mask = 0xffffffff
+        if names is not None:
+            mask = 0
+            for k in range(NKINDS):
+                nm = self.L.koral_hip_grid_op_name(k)
+                if nm and nm.decode() in names:
+                    mask |= 1 << k
+        check(self.L.koral_hip_grid_timing_mask(self.h, mask), "grid_timing_mask")
 
     def timing(self, enable=True, read=True):
         """Per-operator launch timing.  Returns {name: (ms, threads, launches)}
