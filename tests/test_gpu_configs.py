@@ -104,3 +104,32 @@ def test_grmhd_torus_full_size_step(oracle):
     assert np.array_equal(fd, S.flags)
     assert np.array_equal(np.asarray(c_o), dg.counters)
     assert abs(dg.dt - 1. / tm) <= 1e-12 / tm
+
+
+def test_implicit_batch_full_size(oracle):
+    """The batched implicit solve of the full 2^22-cell workload (the
+    benchmark's seed) on the device; a random sample of 2^15 cells solved by
+    the oracle from the same inputs agrees on attempt indices and
+    iteration counts, and on the primitives to the per-step bar."""
+    import torch
+    from koral_amd import cell
+    P = KP.params_implicit_batch()
+    n = 1 << 22
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=3)
+    Pp, g, G = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20)
+    U = cell.p2u(P, Pp, g, G)
+    fl, info = cell.implicit(P, U, Pp, g, G, torch.from_numpy(dt).cuda())
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(11)
+    idx = np.sort(rng.choice(n, 1 << 15, replace=False))
+    u0 = cell.to_host(cell.p2u(P, cell.to_dev(pp[idx]), cell.to_dev(g20[idx]), cell.to_dev(G20[idx])))
+    uo_r, po_r, fl_r, info_r = oracle.implicit(P, u0, pp[idx], g20[idx], G20[idx], dt[idx], nthreads=16)
+    info = info.cpu().numpy().T[idx]
+    po = cell.to_host(Pp)[idx]
+    same = np.all(info == info_r, axis=1)
+    ok = same & (info[:, 1] >= 0)
+    rel = (np.abs(po[ok] - po_r[ok]) / (np.abs(po[ok]) + np.abs(po_r[ok]) + 1e-300)).max(1)
+    _say("  2^22 cells, sample %d: info identical %.5f, solved %.4f, max rel %.3e, 99.9%% %.3e" %
+         (len(idx), same.mean(), (info[:, 1] >= 0).mean(), rel.max(), np.quantile(rel, 0.999)))
+    assert same.mean() > 0.999
+    assert np.quantile(rel, 0.999) < 1e-10
