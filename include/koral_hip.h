@@ -151,6 +151,11 @@ int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, cons
    (RADIMPMAXITER when maxiter <= 0 or larger) run the unchanged six-attempt chain from the
    old state.  info[1*n+i] = 6 marks a cell solved by the learned attempt; info[5*n+i]
    counts the iterations of all attempts. */
+/* The same first attempt from starts the caller provides: gxg [4][n] (u, u~^1..3) and gxr
+   [4][n] (E, F~^1..3, may be NULL for gas primitives only), device arrays. */
+int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, double *pp, const double *g20,
+                                  const double *G20, const double *dt, double global_time, const double *gxg,
+                                  const double *gxr, int maxiter, int *flags, int *info, void *stream);
 int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
                                      double *uu, double *pp, const double *g20, const double *G20,
                                      const double *dt, double global_time, int maxiter, int *flags, int *info,

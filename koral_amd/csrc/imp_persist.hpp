@@ -22,10 +22,13 @@
 // conservation has no valid radiation state or E^ < RADIMPLICITTHRESHOLD u
 // (the choice rad.c:1381 makes on the old state, applied to the predicted
 // one), and on the gas primitives from the gas guess otherwise; without a
-// radiation guess every cell takes the gas primitives.  A cell the pass does
-// not solve within its iteration budget enters attempt 0 of the plain chain
-// unchanged, so the guess can change the iteration count but not what the
-// chain would do without it.  A cell solved by that pass reports attempt 6.
+// radiation guess every cell takes the gas primitives.  With both guesses a
+// cell the first learned try does not solve within its iteration budget
+// tries the other primitive set from its guess (as the plain chain falls
+// back between primitive sets).  A cell the learned tries do not solve enters
+// attempt 0 of the plain chain unchanged, so the guess can change the
+// iteration count but not what the chain would do without it.  A cell
+// solved by a learned try reports attempt 6.
 //
 // Src provides, for a cell index c:
 //   time()                         global time of the solve (device side)
@@ -43,17 +46,25 @@ namespace kd {
 // Device buffers of the passes: two list pairs (ping-pong, each n entries per
 // primitive kind) and IMP_NCTR counters.  Counter 4 a + k: k = 0, 1 the list
 // lengths for attempt a (MHD, RAD); k = 2, 3 the work counters of its two
-// kernels.  The learned pass reads list pair 1 (free until attempt 0 writes
-// its failures there) with lengths IMP_LEARN_LEN + k and work counters
-// IMP_LEARN_Q + k.
+// kernels.  The first learned try reads list pair 1 (free until attempt 0
+// writes its failures there) with lengths IMP_LEARN_LEN + k and work
+// counters IMP_LEARN_Q + k; the second try reads the lists second[k] with
+// lengths IMP_SECOND_LEN + k and work counters IMP_SECOND_Q + k (counters
+// 24-27 are otherwise unused: attempt 5 appends nothing).
 constexpr int IMP_NCTR = 32;
+constexpr int IMP_SECOND_LEN = 24;
+constexpr int IMP_SECOND_Q = 26;
 constexpr int IMP_LEARN_LEN = 28;
 constexpr int IMP_LEARN_Q = 30;
 constexpr int IMP_LEARNED_ATTEMPT = 6;
 struct ImpBufs {
   unsigned *list[2][2];
+  unsigned *second[2];  // lists of the second learned try (learned solves only)
   unsigned *ctr;
 };
+
+// pass modes of k_imp_pass
+enum { IMP_CHAIN = 0, IMP_LEARN1 = 1, IMP_LEARN2 = 2 };
 
 KD_INL int imp_other(int wp) { return wp == P_RAD ? P_MHD : P_RAD; }
 
@@ -128,7 +139,7 @@ __global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Sr
 
 // attempt a on the cells of one list, all iterating on primitives WP with
 // equation WE in frame WF
-template <int MAG, int RAD, int WP, int WE, int WF, bool LEARN, class Src>
+template <int MAG, int RAD, int WP, int WE, int WF, int MODE, class Src>
 __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a, ImpBufs b, int *counts,
                                                   const double *gxg, const double *gxr, long n_all)
 {
@@ -138,13 +149,20 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
   const int kind = (WP == P_MHD) ? 0 : 1;
   // the learned pass takes every cell in index order and hands failures to
   // attempt 0; attempt a takes its list and hands failures to attempt a + 1
-  const unsigned *list = LEARN ? b.list[1][kind] : b.list[a & 1][kind];
-  unsigned *out0 = LEARN ? b.list[0][0] : b.list[(a + 1) & 1][0];
-  unsigned *out1 = LEARN ? b.list[0][1] : b.list[(a + 1) & 1][1];
-  const long n = LEARN ? (long)b.ctr[IMP_LEARN_LEN + kind] : (long)b.ctr[4 * a + kind];
-  unsigned *queue = LEARN ? &b.ctr[IMP_LEARN_Q + kind] : &b.ctr[4 * a + 2 + kind];
-  unsigned *out_len = LEARN ? &b.ctr[0] : &b.ctr[4 * (a + 1)];
-  int next_kind = LEARN ? 0 : ((imp_next_wp(a, WP) == P_MHD) ? 0 : 1);
+  constexpr bool LEARN = MODE != IMP_CHAIN;
+  // the first learned try hands failures to the second try on the other
+  // primitives when there is one (b.second set), else to attempt 0; the
+  // second try hands them to attempt 0; attempt a hands them to a + 1
+  const bool to_second = MODE == IMP_LEARN1 && b.second[0] != nullptr;
+  const unsigned *list = MODE == IMP_LEARN1 ? b.list[1][kind] : (MODE == IMP_LEARN2 ? b.second[kind] : b.list[a & 1][kind]);
+  unsigned *out0 = to_second ? b.second[0] : (LEARN ? b.list[0][0] : b.list[(a + 1) & 1][0]);
+  unsigned *out1 = to_second ? b.second[1] : (LEARN ? b.list[0][1] : b.list[(a + 1) & 1][1]);
+  const long n = MODE == IMP_LEARN1 ? (long)b.ctr[IMP_LEARN_LEN + kind]
+                                    : (MODE == IMP_LEARN2 ? (long)b.ctr[IMP_SECOND_LEN + kind] : (long)b.ctr[4 * a + kind]);
+  unsigned *queue = MODE == IMP_LEARN1 ? &b.ctr[IMP_LEARN_Q + kind]
+                                       : (MODE == IMP_LEARN2 ? &b.ctr[IMP_SECOND_Q + kind] : &b.ctr[4 * a + 2 + kind]);
+  unsigned *out_len = to_second ? &b.ctr[IMP_SECOND_LEN] : (LEARN ? &b.ctr[0] : &b.ctr[4 * (a + 1)]);
+  int next_kind = to_second ? 1 - kind : (LEARN ? 0 : ((imp_next_wp(a, WP) == P_MHD) ? 0 : 1));
   bool active = false, fresh = false, exhausted = n == 0;
   unsigned c = 0, c_push = 0;
   bool push = false;
@@ -250,7 +268,7 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
           src.done_fail(c);
           n_bad++;
         } else {
-          if (LEARN) next_kind = (imp_startwith<MAG, RAD>(P, pp00, g) == P_MHD) ? 0 : 1;
+          if (LEARN && !to_second) next_kind = (imp_startwith<MAG, RAD>(P, pp00, g) == P_MHD) ? 0 : 1;
           push = true;
           c_push = c;
         }
@@ -297,14 +315,14 @@ int resident_blocks(K kernel, int bs, long n, unsigned *blocks)
   return 0;
 }
 
-template <int MAG, int WP, int WE, int WF, bool LEARN = false, class Src>
+template <int MAG, int WP, int WE, int WF, int MODE = IMP_CHAIN, class Src>
 int imp_launch_pass(const koral_params &P, const Src &src, long n, int a, const ImpBufs &b, int *counts,
                     hipStream_t s, const double *gxg = nullptr, const double *gxr = nullptr)
 {
   unsigned blocks = 0;
-  int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src>, 256, n, &blocks);
+  int rc = resident_blocks(k_imp_pass<MAG, 1, WP, WE, WF, MODE, Src>, 256, n, &blocks);
   if (rc) return rc;
-  k_imp_pass<MAG, 1, WP, WE, WF, LEARN, Src><<<blocks, 256, 0, s>>>(P, src, a, b, counts, gxg, gxr, n);
+  k_imp_pass<MAG, 1, WP, WE, WF, MODE, Src><<<blocks, 256, 0, s>>>(P, src, a, b, counts, gxg, gxr, n);
   return 0;
 }
 
@@ -323,8 +341,15 @@ int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs
     k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxg, gxr);
     koral_params Pl = P;
     if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
-    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, counts, s, gxg, gxr);
-    if (!rc && gxr) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, true>(Pl, src, n, 0, b, counts, s, gxg, gxr);
+    // without a radiation guess there is no second try: b.second must then be null
+    ImpBufs bl = b;
+    if (!gxr) bl.second[0] = bl.second[1] = nullptr;
+    rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, IMP_LEARN1>(Pl, src, n, 0, bl, counts, s, gxg, gxr);
+    if (!rc && gxr) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, IMP_LEARN1>(Pl, src, n, 0, bl, counts, s, gxg, gxr);
+    if (!rc && gxr && bl.second[0]) {
+      rc = imp_launch_pass<MAG, P_MHD, ENERGYEQ, LAB, IMP_LEARN2>(Pl, src, n, 0, bl, counts, s, gxg, gxr);
+      if (!rc) rc = imp_launch_pass<MAG, P_RAD, ENERGYEQ, LAB, IMP_LEARN2>(Pl, src, n, 0, bl, counts, s, gxg, gxr);
+    }
   } else {
     k_imp_classify<MAG, 1, Src><<<nb, 256, 0, s>>>(P, src, n, b);
   }

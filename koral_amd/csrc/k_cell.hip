@@ -249,13 +249,17 @@ int learned_buffers(ImpSet *s)
 {
   if (s->lmem) return 0;
   const size_t cap = (size_t)s->cap;
-  int rc = check_hip(hipMalloc(&s->lmem, cap * (11 * sizeof(float) + 4 * sizeof(float) + 8 * sizeof(double))),
-                     "hipMalloc learned buffers");
+  int rc = check_hip(
+      hipMalloc(&s->lmem, cap * (11 * sizeof(float) + 4 * sizeof(float) + 8 * sizeof(double) + 2 * sizeof(unsigned))),
+      "hipMalloc learned buffers");
   if (rc) return rc;
   s->gxg = (double *)s->lmem;
   s->gxr = s->gxg + 4 * cap;
   s->feat = (float *)(s->gxr + 4 * cap);
   s->y = s->feat + 11 * cap;
+  // work lists of the second learned try
+  s->b.second[0] = (unsigned *)(s->y + 4 * cap);
+  s->b.second[1] = s->b.second[0] + cap;
   return 0;
 }
 
@@ -363,6 +367,37 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   if (rc) return rc;
   if (rc2) return rc2;
   return check_launch("koral_hip_implicit_batch");
+}
+
+int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, double *pp, const double *g20,
+                                  const double *G20, const double *dt, double global_time, const double *gxg,
+                                  const double *gxr, int maxiter, int *flags, int *info, void *stream)
+{
+  if (check_params(P) || n < 0 || !gxg) return set_error(KORAL_E_ARG, "implicit_batch_from: bad arguments");
+  if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
+  if (n == 0) return 0;
+  if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "implicit batch larger than 2^31 cells");
+  hipStream_t s = (hipStream_t)stream;
+  ImpSet *set = nullptr;
+  int rc = acquire_set(n, s, &set);
+  if (rc) return rc;
+  // the second try on the other primitives needs the lists made with the learned buffers
+  if (gxr) rc = learned_buffers(set);
+  if (rc) {
+    (void)release_set(set, s);
+    return rc;
+  }
+  if (P->mag) {
+    BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<1>(*P, src, n, set->b, nullptr, s, gxg, gxr, maxiter);
+  } else {
+    BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<0>(*P, src, n, set->b, nullptr, s, gxg, gxr, maxiter);
+  }
+  int rc2 = release_set(set, s);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  return check_launch("koral_hip_implicit_batch_from");
 }
 
 int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,

@@ -49,6 +49,7 @@ def _sig(L):
         "koral_hip_mlp_guess": [vp, l, i, vp, vp, vp, vp],
         "koral_hip_implicit_batch_learned": [vp, vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp,
                                              vp],
+        "koral_hip_implicit_batch_from": [vp, l, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, i, vp, vp, vp],
     }.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -237,6 +238,20 @@ def implicit_learned(P, gas, rad, uu, pp, g20, G20, dt, global_time=0.0, maxiter
                                                  _ptr(_chk(G20)), _ptr(_chk(dt)), float(global_time),
                                                  int(maxiter), _ptr(flags), _ptr(info), _stream()),
           "implicit_learned")
+    return flags, info
+
+
+def implicit_from(P, gxg, gxr, uu, pp, g20, G20, dt, global_time=0.0, maxiter=0):
+    """koral_hip_implicit_batch_from: the learned first attempt from the
+    given starts gxg, gxr [4][n] (gxr may be None)."""
+    L = _sig(lib())
+    n = pp.shape[1]
+    flags = torch.zeros((7, n), dtype=torch.int32, device=pp.device)
+    info = torch.zeros((6, n), dtype=torch.int32, device=pp.device)
+    check(L.koral_hip_implicit_batch_from(ctypes.byref(P), n, _ptr(_chk(uu)), _ptr(_chk(pp)), _ptr(_chk(g20)),
+                                          _ptr(_chk(G20)), _ptr(_chk(dt)), float(global_time),
+                                          _ptr(_chk(gxg)), _ptr(_chk(gxr)) if gxr is not None else None,
+                                          int(maxiter), _ptr(flags), _ptr(info), _stream()), "implicit_from")
     return flags, info
 
 

@@ -604,9 +604,11 @@ static int learned_kind(const oracle_params *P, const double *uu0, const double 
    equation, lab frame) with at most maxiter iterations: from the gas start
    gxg on the gas primitives, or, when a radiation start gxr (E, F~^1..3) is
    given and learned_kind chooses it, from gxr on the radiation primitives.
-   A cell it solves reports attempt 6, otherwise the unchanged six-attempt
-   chain runs from the old state and total_iters includes the extra attempt.
-   The learned initial guess has no reference counterpart. */
+   With gxr given, a cell that attempt does not solve tries the other
+   primitive set from its start the same way.  A cell a learned try solves
+   reports attempt 6, otherwise the unchanged six-attempt chain runs from the
+   old state and total_iters includes the learned tries.  The learned initial
+   guess has no reference counterpart. */
 int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, const o_geom *g, double dt,
                                double global_time, int *flags, o_imp_info *info, const double *gxg,
                                const double *gxr, int maxiter)
@@ -637,6 +639,11 @@ int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, c
     params[2] = O_LAB;
     ret = o_solve_implicit_4dprim_from(P, uu00, pp00, g, dt, params, ppn, global_time, &it, gxg, gxr, maxiter);
     total += it;
+    if (ret != 0 && gxr) {
+      params[0] = (params[0] == O_RAD) ? O_MHD : O_RAD;
+      ret = o_solve_implicit_4dprim_from(P, uu00, pp00, g, dt, params, ppn, global_time, &it, gxg, gxr, maxiter);
+      total += it;
+    }
     if (ret == 0) a = 6;
   }
   for (; ret != 0 && a < 6; a++) {
