@@ -264,7 +264,7 @@ def bench_learned(args):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "learned implicit solve: 2 x (features + BF16 inference + guess) + classify + "
-                               "14 k_imp_pass launches (one batch)",
+                               "16 k_imp_pass launches (one batch)",
                      "kernel_ms": k_learn * 1e3,
                      "algorithmic_flops_per_launch": learned_flops * n,
                      "flops_basis": "counted FP64 operations per plain solve / plain mean iterations x "

@@ -143,13 +143,14 @@ int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, v
 int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, const float *Y, double *gx,
                         void *stream);
 /* koral_hip_implicit_batch with a first attempt from the learned guess (energy equation, lab
-   frame) under the unchanged convergence tolerance.  With a radiation network (rad may be
-   NULL) a cell iterates the radiation primitives from its guess when the gas guess,
-   completed by conservation, has E^ < RADIMPLICITTHRESHOLD u or no valid radiation state
-   (the criterion of rad.c:1381 applied to the predicted state; its gas part starts from the
-   gas guess), and the gas primitives from the gas guess otherwise.  Cells the attempt does not solve within maxiter iterations
-   (RADIMPMAXITER when maxiter <= 0 or larger) run the unchanged six-attempt chain from the
-   old state.  info[1*n+i] = 6 marks a cell solved by the learned attempt; info[5*n+i]
+   frame) under the unchanged convergence tolerance.  Without a radiation network (rad may be
+   NULL) every cell iterates the gas primitives from the gas guess.  With one, a cell first
+   iterates the primitives attempt 0 of the chain would (rad.c:1381 on the old state): the
+   radiation primitives from the radiation guess (gas part from the gas guess) or the gas
+   primitives from the gas guess; a cell that try does not solve tries the other primitives
+   the same way.  Each try stops after maxiter iterations (RADIMPMAXITER when maxiter <= 0
+   or larger).  Cells the learned tries do not solve run the unchanged six-attempt chain
+   from the old state.  info[1*n+i] = 6 marks a cell solved by a learned try; info[5*n+i]
    counts the iterations of all attempts. */
 /* The same first attempt from starts the caller provides: gxg [4][n] (u, u~^1..3) and gxr
    [4][n] (E, F~^1..3, may be NULL for gas primitives only), device arrays. */

@@ -16,13 +16,15 @@
 // per attempt, so no lane waits for the slowest cell of its wave.
 //
 // With a learned initial guess a first pass runs before the chain: the
-// energy-equation lab-frame solve from a start given per cell.  The learned
-// classification puts a cell on the radiation primitives, starting from the
-// radiation network's guess, when the gas network's guess completed by
-// conservation has no valid radiation state or E^ < RADIMPLICITTHRESHOLD u
-// (the choice rad.c:1381 makes on the old state, applied to the predicted
-// one), and on the gas primitives from the gas guess otherwise; without a
-// radiation guess every cell takes the gas primitives.  With both guesses a
+// energy-equation lab-frame solve from a start given per cell.  With a
+// radiation guess the learned classification takes the primitives of the
+// chain's attempt 0 (the old state's E^ < RADIMPLICITTHRESHOLD u choice,
+// rad.c:1381): the radiation primitives from the radiation network's guess
+// (the gas part from the gas guess), or the gas primitives from the gas
+// guess; without a radiation guess every cell takes the gas primitives.
+// Iterations on the gas primitives are much cheaper (no inner gas inversion
+// per residual), so a cell whose old state favours them tries them first
+// even where it will need the radiation primitives.  With both guesses a
 // cell the first learned try does not solve within its iteration budget
 // tries the other primitive set from its guess (as the plain chain falls
 // back between primitive sets).  A cell the learned tries do not solve enters
@@ -109,28 +111,23 @@ __global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, l
   wave_append(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
 }
 
-// learned classification: the primitive kind of every cell's learned
-// attempt (see above); gxr may be null
+// learned classification: the primitive kind of every cell's first learned
+// try (see above); gxr may be null
 template <int MAG, class Src>
 __global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Src src, long n, ImpBufs b,
-                                                              const double *__restrict__ gxg,
                                                               const double *__restrict__ gxr)
 {
   constexpr int NV = Lay<MAG, 1>::NV;
-  constexpr int EE = Lay<MAG, 1>::EE;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int wp = 0;
   if (c < n) {
     src.init(c);
     wp = P_MHD;
     if (gxr) {
-      double uu00[NV], pp00[NV], dt, uu[NV];
+      double uu00[NV], pp00[NV], dt;
       Geom g;
       src.load(c, uu00, pp00, g, dt);
-#pragma unroll
-      for (int i = 0; i < 4; i++) pp00[1 + i] = gxg[i * n + c];
-      const int r = complete_state<MAG, 1, P_MHD>(P, pp00, uu, uu00, g, false);
-      if (r != 0 || imp_startwith<MAG, 1>(P, pp00, g) == P_RAD) wp = P_RAD;
+      wp = imp_startwith<MAG, 1>(P, pp00, g);
     }
   }
   wave_append(wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
@@ -338,7 +335,7 @@ int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
   int rc = 0;
   if (gxg) {
-    k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxg, gxr);
+    k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxr);
     koral_params Pl = P;
     if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
     // without a radiation guess there is no second try: b.second must then be null

@@ -569,43 +569,13 @@ int o_solve_implicit_lab(const oracle_params *P, double *uu, double *pp, const o
   return o_solve_implicit_lab_guess(P, uu, pp, g, dt, global_time, flags, info, NULL, NULL, 0);
 }
 
-/* Primitive kind of the learned attempt: the gas start gxg (u, u~^1..3)
-   completed by conservation as the Newton update does (rad.c:880-950); the
-   radiation primitives when that has no valid radiation state or
-   E^ < RADIMPLICITTHRESHOLD u (the rad.c:1381 choice on the predicted
-   state), the gas primitives otherwise. */
-static int learned_kind(const oracle_params *P, const double *uu0, const double *pp00, const o_geom *g,
-                        const double *gxg)
-{
-  int nv = P->nv, ee = ee_of(P);
-  double pp[O_NVMAX], uu[O_NVMAX];
-  for (int i = 0; i < nv; i++) pp[i] = pp00[i];
-  for (int i = 0; i < 4; i++) pp[1 + i] = gxg[i];
-  double ur[4] = {0., pp[2], pp[3], pp[4]}, uc[4];
-  o_relr_to_4vel(ur, uc, NULL, g);
-  pp[0] = (uu0[0]) / g->gdet / uc[0];
-  pp[5] = o_calc_Sfromu(P, pp[0], pp[1]);
-  o_p2u(P, pp, uu, g);
-  uu[5] = uu0[5] - (uu[ee] - uu0[ee]);
-  uu[ee] = uu0[ee] - (uu[1] - uu0[1]);
-  uu[ee + 1] = uu0[ee + 1] - (uu[2] - uu0[2]);
-  uu[ee + 2] = uu0[ee + 2] - (uu[3] - uu0[3]);
-  uu[ee + 3] = uu0[ee + 3] - (uu[4] - uu0[4]);
-  int corr = 0;
-  int r = o_u2p_rad(P, uu, pp, g, &corr);
-  if (corr > 0) r = -1;
-  if (r != 0) return O_RAD;
-  double Rtt;
-  o_calc_ff_Rtt(P, pp, g, &Rtt, NULL);
-  return (-Rtt < P->radimplicitthreshold * pp[1]) ? O_RAD : O_MHD;
-}
-
 /* solve_implicit_lab preceded, when gxg is given, by one attempt (energy
    equation, lab frame) with at most maxiter iterations: from the gas start
    gxg on the gas primitives, or, when a radiation start gxr (E, F~^1..3) is
-   given and learned_kind chooses it, from gxr on the radiation primitives.
-   With gxr given, a cell that attempt does not solve tries the other
-   primitive set from its start the same way.  A cell a learned try solves
+   given and the old state has E^ < RADIMPLICITTHRESHOLD u (the choice of
+   attempt 0, rad.c:1381), from gxr on the radiation primitives with the gas
+   part from gxg.  With gxr given, a cell that attempt does not solve tries
+   the other primitive set from its start the same way.  A cell a learned try solves
    reports attempt 6, otherwise the unchanged six-attempt chain runs from the
    old state and total_iters includes the learned tries.  The learned initial
    guess has no reference counterpart. */
@@ -634,7 +604,7 @@ int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, c
   int ret = -1, total = 0, it = 0, a = 0;
   int params[4] = {0, 0, 0, 0};
   if (gxg) {
-    params[0] = (gxr && learned_kind(P, uu00, pp00, g, gxg) == O_RAD) ? O_RAD : O_MHD;
+    params[0] = gxr ? startwith : O_MHD;
     params[1] = O_ENERGYEQ;
     params[2] = O_LAB;
     ret = o_solve_implicit_4dprim_from(P, uu00, pp00, g, dt, params, ppn, global_time, &it, gxg, gxr, maxiter);
