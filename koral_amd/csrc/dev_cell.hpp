@@ -16,6 +16,30 @@
 
 namespace kd {
 
+// Appends v to list for the lanes with pred (order within a wave by lane,
+// across waves by arrival): the wave ballots its lanes, one lane reserves
+// the wave's entries with one atomic add.  Every active lane of the wave
+// calls it.
+#ifdef KD_HOST_CHECK
+inline void append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
+{
+  if (pred) list[(*len)++] = v;
+}
+#else
+__device__ __forceinline__ void append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
+{
+  const unsigned long long m = __ballot(pred);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(len, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+#endif
+
+
 constexpr double SMALL = 1.e-50;
 constexpr double BIG = 1. / SMALL;
 constexpr double GAMMASMALLLIMIT = (1.0 - 1E-10);
@@ -627,8 +651,14 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
   return ret;
 }
 
-// check_floors_mhd, u2p.c:492-695
-template <int MAG, int RAD>
+// check_floors_mhd, u2p.c:492-695.  With DEFER a state that needs the b^2/rho
+// floor (ZAMO mass injection and re-inversion, u2p.c:600-634) is left for a
+// later pass: the function returns FLOORS_DEFERRED at once, with pp partly
+// floored, and the caller redoes the state with DEFER false.  Keeping the
+// two inlined Newton inversions of that rare branch out of a kernel's
+// common path is what DEFER is for.
+constexpr int FLOORS_DEFERRED = 1;
+template <int MAG, int RAD, bool DEFER = false>
 KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -654,6 +684,7 @@ KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
     lower(bcon, bcov, g);
     double magpre = dot4(bcon, bcov) / 2.;
     if (magpre > P.b2rhoratiomax * pp[0]) {
+      if (DEFER) return FLOORS_DEFERRED;
       double alp = 1. / sqrt(-g.G[0][0]);
       double ncov[4] = {-alp, 0., 0., 0.}, etacon[4];
       raise_(ncov, etacon, g);

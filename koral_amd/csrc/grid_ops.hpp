@@ -6,6 +6,9 @@
 // calc_u2p with cell_fixup, op_implicit, RK2IMEX combinations) is written
 // once against an executor Ex that provides
 //   ex.each(op)          run op(i) for every i
+//   ex.fork(k), ex.each_on(k, op), ex.join(k)   the same on side stream k,
+//                        ordered after the main stream's work at fork and
+//                        before its later work at join
 //   ex.max(op, out)      *out = max(*out, max_i op(i))     (op(i) >= 0)
 //   ex.count(op, out3)   out3[k] += sum_i op(i).v[k], k < 3
 //   ex.implicit_cells(op, out3)   the implicit sweep OpImplicit with its
@@ -22,7 +25,7 @@ namespace kd {
 // Operator kinds (per-kind launch timing and names in the C-ABI)
 enum OpKind {
   K_P2U, K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_FLUX, K_EMF, K_CT, K_UPDATE,
-  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_NKINDS
+  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_RECON, K_FLUX_FLOOR, K_NKINDS
 };
 
 struct Box {
@@ -340,103 +343,199 @@ KD_INL int dir_index(int ix, int iy, int iz)
   return D == 0 ? ix : (D == 1 ? iy : iz);
 }
 
-// Face states of the face between cells -1 and 0 of direction D.  Cells are
-// addressed as offsets from the face's right cell, so no coordinate array is
-// indexed by the direction at run time.
-template <int MAG, int RAD, int ORDER, int D>
-KD_INL void recon_face(const koral_params &P, const DGrid &G, const double *__restrict__ p, int ix, int iy, int iz,
-                       double *pL, double *pR)
+// Reconstruction of one variable at a face of direction D from the cells
+// -3..+2 around it (finite.c:694-1014, avg2point finite.c:90-289): q points
+// at the variable in cell 0 (the right cell of the face), st is the cell
+// stride of direction D, dx the cell widths (PPM only).  l is the state on
+// the left side of the face (right edge of cell -1), r on the right side.
+template <int ORDER>
+KD_INL void recon_var(const koral_params &P, const double *__restrict__ q, long st, const double *dx, double &l,
+                      double &r)
 {
-  constexpr int NV = Lay<MAG, RAD>::NV;
-  const long n = G.ncell;
-  const long c0 = cidx(G, ix, iy, iz), st = dir_stride<D>(G);
   if (ORDER == 1) {
     const double th = P.minmod_theta;
-#pragma unroll
-    for (int iv = 0; iv < NV; iv++) {
-      const double *q = p + iv * n + c0;
-      double um1 = q[-2 * st], u0 = q[-st], up1 = q[0];
-      pL[iv] = u0 + .5 * minmod3(th * (up1 - u0), .5 * (up1 - um1), th * (u0 - um1));
-      double vm1 = u0, v0 = up1, vp1 = q[st];
-      pR[iv] = v0 - .5 * minmod3(th * (vp1 - v0), .5 * (vp1 - vm1), th * (v0 - vm1));
-    }
+    double um1 = q[-2 * st], u0 = q[-st], up1 = q[0];
+    l = u0 + .5 * minmod3(th * (up1 - u0), .5 * (up1 - um1), th * (u0 - um1));
+    double vm1 = u0, v0 = up1, vp1 = q[st];
+    r = v0 - .5 * minmod3(th * (vp1 - v0), .5 * (vp1 - vm1), th * (v0 - vm1));
     return;
   }
-  // cells -3..+2 relative to the face
-  double dx[6];
-  const int id = dir_index<D>(ix, iy, iz);
+  double v[6];
 #pragma unroll
-  for (int o = 0; o < 6; o++) dx[o] = (ORDER == 2) ? size_x(G, id + o - 3, D) : 0.;
-#pragma unroll
-  for (int iv = 0; iv < NV; iv++) {
-    const double *q = p + iv * n + c0;
-    double v[6];
-#pragma unroll
-    for (int o = 0; o < 6; o++) v[o] = q[(o - 3) * st];
-    if (ORDER == 4) {
-      pL[iv] = mp5(v[0], v[1], v[2], v[3], v[4]);  // right face of cell -1
-      pR[iv] = mp5(v[5], v[4], v[3], v[2], v[1]);  // left face of cell 0
-    } else {
-      double ul, ur;
-      ppm(v, dx, ul, ur);
-      pL[iv] = ur;
-      ppm(v + 1, dx + 1, ul, ur);
-      pR[iv] = ul;
-    }
+  for (int o = 0; o < 6; o++) v[o] = q[(o - 3) * st];
+  if (ORDER == 4) {
+    l = mp5(v[0], v[1], v[2], v[3], v[4]);
+    r = mp5(v[5], v[4], v[3], v[2], v[1]);
+  } else {
+    double ul, ur;
+    ppm(v, dx, ul, ur);
+    l = ur;
+    ppm(v + 1, dx + 1, ul, ur);
+    r = ul;
   }
 }
 
+// Face states of one sweep, KORAL's pbL/pbR arrays: both sides of every face
+// of box b, stored compactly by box index, fs[iv][i] (left) and fs[NV + iv][i]
+// (right).  One variable at a time, so the kernel holds few registers and
+// runs at full occupancy; the branchy limiters (MP5) stay out of the
+// register-heavy flux kernel.
 template <int MAG, int RAD, int ORDER, int D>
-struct OpFlux {
-  static constexpr int kind = K_FLUX;
-  static constexpr int waves_per_simd = ORDER == 4 ? 2 : 1;  // MP5 fits two waves with few spills
+struct OpRecon {
+  static constexpr int kind = K_RECON;
   koral_params P;
   DGrid G;
-  const double *p, *ws;
-  double *flb;
+  const double *p;
+  double *fs;
   Box b;
   KD_HD long size() const { return b.size(); }
   KD_INL void operator()(long i) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
-    constexpr int NVMHD = 6 + 3 * MAG;
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    const long n = G.ncell, nb = b.size();
+    const long c0 = cidx(G, ix, iy, iz), st = dir_stride<D>(G);
+    double dx[6];
+    const int id = dir_index<D>(ix, iy, iz);
+#pragma unroll
+    for (int o = 0; o < 6; o++) dx[o] = (ORDER == 2) ? size_x(G, id + o - 3, D) : 0.;
+#pragma unroll 1
+    for (int iv = 0; iv < NV; iv++) {
+      double l, r;
+      recon_var<ORDER>(P, p + iv * n + c0, st, dx, l, r);
+      fs[iv * nb + i] = l;
+      fs[(NV + iv) * nb + i] = r;
+    }
+  }
+};
+
+// Fluxes at the face (ix, iy, iz) of direction D from its floored face
+// states: physical fluxes, conserved states, HLL or LF with the wave speeds
+// of the two cells (f_calc_fluxes_at_faces finite.c:1297-1415)
+template <int MAG, int RAD, int D>
+KD_INL void face_flux(const koral_params &P, const DGrid &G, const double *__restrict__ ws, double *flb, int ix,
+                      int iy, int iz, const double *pL, const double *pR, const Geom &g)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  constexpr int NVMHD = 6 + 3 * MAG;
+  double fL[NV], fR[NV], uL[NV], uR[NV];
+  flux_prime<MAG, RAD>(P, pL, D + 1, g, fL);
+  flux_prime<MAG, RAD>(P, pR, D + 1, g, fR);
+  kd::p2u<MAG, RAD>(P, pL, uL, g);
+  kd::p2u<MAG, RAD>(P, pR, uR, g);
+  const long n = G.ncell;
+  const long cR = cidx(G, ix, iy, iz), cL = cR - dir_stride<D>(G);
+  const long f = fidx(G, D, ix, iy, iz);
+  const long nf = D == 0 ? G.nfx : (D == 1 ? G.nfy : G.nfz);
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) {
+    const int r = (RAD && iv >= NVMHD) ? 1 : 0;
+    double ag = mx(ws[(12 + 3 * r + D) * n + cR], ws[(12 + 3 * r + D) * n + cL]);
+    double al = mn(ws[(6 * r + 2 * D) * n + cR], ws[(6 * r + 2 * D) * n + cL]);
+    double ar = mx(ws[(6 * r + 2 * D + 1) * n + cR], ws[(6 * r + 2 * D + 1) * n + cL]);
+    double F;
+    if (P.fluxmethod == 0) {
+      F = .5 * (fR[iv] + fL[iv] - ag * (uR[iv] - uL[iv]));
+    } else {
+      if (al > 0.)
+        F = fL[iv];
+      else if (ar < 0.)
+        F = fL[iv];
+      else
+        F = (-al * fR[iv] + ar * fL[iv] + al * ar * (uR[iv] - uL[iv])) / (ar - al);
+    }
+    flb[iv * nf + f] = F;
+  }
+}
+
+// face states of face i of a sweep box, stored by OpRecon
+template <int NV>
+KD_INL void load_face_states(const double *__restrict__ fs, long nb, long i, double *pL, double *pR)
+{
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) {
+    pL[iv] = fs[iv * nb + i];
+    pR[iv] = fs[(NV + iv) * nb + i];
+  }
+}
+
+// Fluxes at the faces of box b from the face states of OpRecon, with the
+// face floors (check_floors_mhd, finite.c:801-993) on both states.  A face
+// whose states need the b^2/rho floor (mass injection and a Newton
+// re-inversion, rare and long) is appended to list (length *cnt, cleared
+// before the sweep) and left to OpFluxFloor, so the common path carries no
+// inversion and those few faces run in a handful of waves.
+template <int MAG, int RAD, int D>
+struct OpFlux {
+  static constexpr int kind = K_FLUX;
+  static constexpr int waves_per_simd = 2;
+  koral_params P;
+  DGrid G;
+  const double *fs, *ws;
+  double *flb;
+  unsigned *list, *cnt;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
     int ix, iy, iz;
     b.at(i, ix, iy, iz);
     double pL[NV], pR[NV];
-    recon_face<MAG, RAD, ORDER, D>(P, G, p, ix, iy, iz, pL, pR);
+    load_face_states<NV>(fs, b.size(), i, pL, pR);
     Geom g;
     geom_face(G, D, ix, iy, g);
-    floors_mhd<MAG, RAD>(P, pL, g);
-    floors_mhd<MAG, RAD>(P, pR, g);
-    double fL[NV], fR[NV], uL[NV], uR[NV];
-    flux_prime<MAG, RAD>(P, pL, D + 1, g, fL);
-    flux_prime<MAG, RAD>(P, pR, D + 1, g, fR);
-    kd::p2u<MAG, RAD>(P, pL, uL, g);
-    kd::p2u<MAG, RAD>(P, pR, uR, g);
-    const long n = G.ncell;
-    const long cR = cidx(G, ix, iy, iz), cL = cR - dir_stride<D>(G);
-    const long f = fidx(G, D, ix, iy, iz);
-    const long nf = D == 0 ? G.nfx : (D == 1 ? G.nfy : G.nfz);
-#pragma unroll
-    for (int iv = 0; iv < NV; iv++) {
-      const int r = (RAD && iv >= NVMHD) ? 1 : 0;
-      double ag = mx(ws[(12 + 3 * r + D) * n + cR], ws[(12 + 3 * r + D) * n + cL]);
-      double al = mn(ws[(6 * r + 2 * D) * n + cR], ws[(6 * r + 2 * D) * n + cL]);
-      double ar = mx(ws[(6 * r + 2 * D + 1) * n + cR], ws[(6 * r + 2 * D + 1) * n + cL]);
-      double F;
-      if (P.fluxmethod == 0) {
-        F = .5 * (fR[iv] + fL[iv] - ag * (uR[iv] - uL[iv]));
-      } else {
-        if (al > 0.)
-          F = fL[iv];
-        else if (ar < 0.)
-          F = fL[iv];
-        else
-          F = (-al * fR[iv] + ar * fL[iv] + al * ar * (uR[iv] - uL[iv])) / (ar - al);
-      }
-      flb[iv * nf + f] = F;
+    const int dl = floors_mhd<MAG, RAD, true>(P, pL, g);
+    const int dr = floors_mhd<MAG, RAD, true>(P, pR, g);
+    const bool defer = dl == FLOORS_DEFERRED || dr == FLOORS_DEFERRED;
+    append_index(defer, (unsigned)i, list, cnt);
+    if (defer) return;
+    face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g);
+  }
+};
+
+// The faces OpFlux left: the same face states, the full face floors, the
+// same fluxes.  FLOOR_THREADS threads stride over the list, whose length is
+// known only on the device.
+constexpr long FLOOR_THREADS = 16384;
+template <int MAG, int RAD, int D>
+struct OpFluxFloor {
+  static constexpr int kind = K_FLUX_FLOOR;
+  static constexpr int waves_per_simd = 2;
+  koral_params P;
+  DGrid G;
+  const double *fs, *ws;
+  double *flb;
+  const unsigned *list, *cnt;
+  Box b;
+  KD_HD long size() const { return FLOOR_THREADS; }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    const long m = (long)*cnt;
+    for (long j = t; j < m; j += FLOOR_THREADS) {
+      const long i = list[j];
+      int ix, iy, iz;
+      b.at(i, ix, iy, iz);
+      double pL[NV], pR[NV];
+      load_face_states<NV>(fs, b.size(), i, pL, pR);
+      Geom g;
+      geom_face(G, D, ix, iy, g);
+      floors_mhd<MAG, RAD>(P, pL, g);
+      floors_mhd<MAG, RAD>(P, pR, g);
+      face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g);
     }
   }
+};
+
+// n (<= 64) counters set to zero by a kernel (safe inside a captured graph)
+struct OpZeroU32 {
+  static constexpr int kind = K_FLUX_FLOOR;
+  unsigned *q;
+  int n;
+  KD_HD long size() const { return n; }
+  KD_INL void operator()(long i) const { q[i] = 0u; }
 };
 
 // ---------------------------------------------------------------------------
@@ -807,6 +906,9 @@ struct GridBufs {
   double *p, *u;
   signed char *fl;
   double *ws, *flb[3], *emf;
+  double *fs[3];      // face states of the sweep of each direction, 2 NV x face_states_cap(G)
+  unsigned *flist[3];  // faces of each sweep left to the b^2/rho floor pass, face_states_cap(G)
+  unsigned *fcnt;      // their counts [3]
   double *pn, *un;
   int *mark;
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
@@ -816,6 +918,14 @@ struct GridBufs {
   unsigned *imp_list[2][2];  // work lists of the implicit passes (device executor)
   unsigned *imp_ctr;         // their counters
 };
+
+// Faces per sweep box bound: a sweep of direction d covers n_d + 1 faces
+// along d and at most n_k + 2 across (flux-CT needs one ghost layer)
+inline long face_states_cap(const DGrid &G)
+{
+  auto w = [](int n) { return n == 1 ? 1L : (long)n + 2; };
+  return w(G.nx) * w(G.ny) * w(G.nz);
+}
 
 // Ghost-inclusive layout of a grid description (a collapsed dimension,
 // n = 1, has no ghosts).  Table pointers are left for the caller.
@@ -915,6 +1025,7 @@ void explicit_op(Ex &ex, const GridBufs &s)
   tstepden(ex, s);
   const int tt = P.mag ? 1 : 0;
   const int n[3] = {G.nx, G.ny, G.nz};
+  ex.each(OpZeroU32{s.fcnt, 3});
   for (int d = 0; d < 3; d++) {
     if (n[d] == 1) continue;
     int lo[3], cnt[3];
@@ -930,11 +1041,18 @@ void explicit_op(Ex &ex, const GridBufs &s)
       auto sweep = [&](auto dd) {
         constexpr int D = decltype(dd)::value;
         if (P.int_order == 4)
-          ex.each(OpFlux<M, R, 4, D>{P, G, s.p, s.ws, s.flb[D], fb});
+          ex.each(OpRecon<M, R, 4, D>{P, G, s.p, s.fs[D], fb});
         else if (P.int_order == 2)
-          ex.each(OpFlux<M, R, 2, D>{P, G, s.p, s.ws, s.flb[D], fb});
+          ex.each(OpRecon<M, R, 2, D>{P, G, s.p, s.fs[D], fb});
         else
-          ex.each(OpFlux<M, R, 1, D>{P, G, s.p, s.ws, s.flb[D], fb});
+          ex.each(OpRecon<M, R, 1, D>{P, G, s.p, s.fs[D], fb});
+        ex.each(OpFlux<M, R, D>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb});
+        // the faces left to the floor pass (a few, each a long Newton
+        // inversion) run on a side stream while the next sweeps proceed
+        if (M) {
+          ex.fork(D);
+          ex.each_on(D, OpFluxFloor<M, R, D>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb});
+        }
       };
       if (d == 0)
         sweep(std::integral_constant<int, 0>{});
@@ -944,6 +1062,9 @@ void explicit_op(Ex &ex, const GridBufs &s)
         sweep(std::integral_constant<int, 2>{});
     });
   }
+  if (P.mag)
+    for (int d = 0; d < 3; d++)
+      if (n[d] > 1) ex.join(d);
   Flx F{s.flb[0], s.flb[1], s.flb[2], G.nfx, G.nfy, G.nfz};
   if (P.mag) {
     ex.each(OpEmf{G, F, s.emf});

@@ -79,19 +79,6 @@ KD_INL int imp_next_wp(int a, int wp)
   return swapped_next ? imp_other(startwith) : startwith;
 }
 
-// append v to list when pred holds; every lane of the wave calls it
-__device__ __forceinline__ void wave_append(bool pred, unsigned v, unsigned *list, unsigned *len)
-{
-  const unsigned long long m = __ballot(pred);
-  if (!m) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(len, (unsigned)__popcll(m));
-  base = __shfl(base, leader);
-  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
-}
-
 // sorts the n cells into the two attempt-0 lists by the primitives the
 // chain starts with
 template <int MAG, int RAD, class Src>
@@ -107,8 +94,8 @@ __global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, l
     src.load(c, uu00, pp00, g, dt);
     wp = imp_startwith<MAG, RAD>(P, pp00, g);
   }
-  wave_append(wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
-  wave_append(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
+  append_index(wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
+  append_index(wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
 }
 
 // learned classification: the primitive kind of every cell's first learned
@@ -130,8 +117,8 @@ __global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Sr
       wp = imp_startwith<MAG, 1>(P, pp00, g);
     }
   }
-  wave_append(wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
-  wave_append(wp == P_RAD, (unsigned)c, b.list[1][1], &b.ctr[IMP_LEARN_LEN + 1]);
+  append_index(wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
+  append_index(wp == P_RAD, (unsigned)c, b.list[1][1], &b.ctr[IMP_LEARN_LEN + 1]);
 }
 
 // attempt a on the cells of one list, all iterating on primitives WP with
@@ -176,8 +163,8 @@ __global__ void __launch_bounds__(256) k_imp_pass(koral_params P, Src src, int a
   const int me = threadIdx.x;
   for (;;) {
     // failures of the last iteration go to the next attempt's list
-    wave_append(push && next_kind == 0, c_push, out0, &out_len[0]);
-    wave_append(push && next_kind == 1, c_push, out1, &out_len[1]);
+    append_index(push && next_kind == 0, c_push, out0, &out_len[0]);
+    append_index(push && next_kind == 1, c_push, out1, &out_len[1]);
     push = false;
     if (!exhausted) {
       const unsigned long long need = __ballot(!active);

@@ -159,8 +159,12 @@ struct HipExec {
   hipStream_t s;
   Timing *tm;
   int rc = 0;  // first launch-configuration error
+  // side streams and their fork/join events (work that may overlap the
+  // main stream, e.g. the few faces of a sweep left to the floor pass)
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork[3] = {nullptr, nullptr, nullptr}, ev_join[3] = {nullptr, nullptr, nullptr};
   template <class Op, class L>
-  void timed(const Op &op, L launch)
+  void timed(const Op &op, L launch, hipStream_t st)
   {
     long n = op.size();
     if (n <= 0) return;
@@ -169,12 +173,39 @@ struct HipExec {
       a = tm->next();
       b = tm->next();
     }
-    if (a && b) (void)hipEventRecord(a, s);
+    if (a && b) (void)hipEventRecord(a, st);
     launch(n);
     if (a && b) {
-      (void)hipEventRecord(b, s);
+      (void)hipEventRecord(b, st);
       tm->recs.push_back(Timing::Rec{Op::kind, n, a, b});
     }
+  }
+  template <class Op, class L>
+  void timed(const Op &op, L launch)
+  {
+    timed(op, launch, s);
+  }
+  void note(hipError_t e, const char *what)
+  {
+    if (e != hipSuccess && !rc) rc = check_hip(e, what);
+  }
+  // side stream k starts after the work launched so far on the main stream
+  void fork(int k)
+  {
+    note(hipEventRecord(ev_fork[k], s), "fork record");
+    note(hipStreamWaitEvent(side[k], ev_fork[k], 0), "fork wait");
+  }
+  // the main stream continues after the work launched so far on side stream k
+  void join(int k)
+  {
+    note(hipEventRecord(ev_join[k], side[k]), "join record");
+    note(hipStreamWaitEvent(s, ev_join[k], 0), "join wait");
+  }
+  template <class Op>
+  void each_on(int k, const Op &op)
+  {
+    hipStream_t st = side[k];
+    timed(op, [&](long n) { k_each<Op><<<blocks_for(n), BS, 0, st>>>(op); }, st);
   }
   template <class Op>
   void each(const Op &op)
@@ -246,6 +277,8 @@ __global__ void k_flags_c2i(long ncell, const signed char *soa, int *aos)
 struct koral_hip_grid {
   GridBufs b{};
   hipStream_t s = nullptr;
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};  // default priority, non-blocking
+  hipEvent_t ev_fork[3] = {nullptr, nullptr, nullptr}, ev_join[3] = {nullptr, nullptr, nullptr};
   Timing timing;
   std::vector<void *> allocs;
   double *stage = nullptr;  // AoS staging for upload/download
@@ -276,11 +309,32 @@ struct koral_hip_grid {
   ~koral_hip_grid()
   {
     drop_graph();
+    for (int k = 0; k < 3; k++) {
+      if (side[k]) (void)hipStreamSynchronize(side[k]);
+      if (side[k]) (void)hipStreamDestroy(side[k]);
+      if (ev_fork[k]) (void)hipEventDestroy(ev_fork[k]);
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+    }
     for (void *q : allocs) (void)hipFree(q);
+  }
+  int make_side()
+  {
+    for (int k = 0; k < 3; k++) {
+      int rc = check_hip(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking), "side stream");
+      if (!rc) rc = check_hip(hipEventCreateWithFlags(&ev_fork[k], hipEventDisableTiming), "fork event");
+      if (!rc) rc = check_hip(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming), "join event");
+      if (rc) return rc;
+    }
+    return 0;
   }
   HipExec ex()
   {
     HipExec e{s, &timing};
+    for (int k = 0; k < 3; k++) {
+      e.side[k] = side[k];
+      e.ev_fork[k] = ev_fork[k];
+      e.ev_join[k] = ev_join[k];
+    }
     e.imp.list[0][0] = b.imp_list[0][0];
     e.imp.list[0][1] = b.imp_list[0][1];
     e.imp.list[1][0] = b.imp_list[1][0];
@@ -397,6 +451,13 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   if (!rc && G.nx > 1) rc = g->alloc(&s.flb[0], nv * G.nfx);
   if (!rc && G.ny > 1) rc = g->alloc(&s.flb[1], nv * G.nfy);
   if (!rc && G.nz > 1) rc = g->alloc(&s.flb[2], nv * G.nfz);
+  for (int d = 0; d < 3 && !rc; d++)
+    if ((d == 0 ? G.nx : d == 1 ? G.ny : G.nz) > 1) {
+      rc = g->alloc(&s.fs[d], 2L * nv * face_states_cap(G));
+      if (!rc) rc = g->alloc(&s.flist[d], face_states_cap(G));
+    }
+  if (!rc) rc = g->alloc(&s.fcnt, 3);
+  if (!rc) rc = g->make_side();
   if (!rc) rc = g->alloc(&s.emf, 3L * (G.nx + 1) * (G.ny + 1) * (G.nz + 1));
   if (!rc) rc = g->alloc(&s.pn, nv * nc);
   if (!rc) rc = g->alloc(&s.un, nv * nc);
@@ -551,7 +612,7 @@ const char *koral_hip_grid_op_name(int kind)
 {
   static const char *names[K_NKINDS] = {"p2u", "bc_faces", "bc_edge", "bc_total", "wavespeeds", "flux",
                                         "emf", "ct_faces", "update", "u2p", "fixup", "fixup_apply",
-                                        "implicit", "comb", "polar"};
+                                        "implicit", "comb", "polar", "recon", "flux_floor"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "";
 }
 
