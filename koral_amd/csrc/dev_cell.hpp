@@ -610,7 +610,12 @@ KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 // high-level inversion and floors
 // ---------------------------------------------------------------------------
 // u2p, u2p.c:176-485 (type 0)
-template <int MAG, int RAD>
+// With DEFER a state whose hot inversion fails is left for a later pass:
+// the function returns U2P_DEFERRED at once (pp then holds a failed
+// iterate) and the caller redoes the state with DEFER false, so the entropy
+// inversion is not inlined into the common path.
+constexpr int U2P_DEFERRED = 1;
+template <int MAG, int RAD, bool DEFER = false>
 KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -632,6 +637,7 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
   }
   u2pret = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 0);
   if (u2pret < 0) {
+    if (DEFER) return U2P_DEFERRED;
     ret = -1;
     u2pret = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 1);
     if (u2pret < 0) ret = -2;
@@ -763,17 +769,22 @@ KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 
 // calc_primitives core (u2p.c:13-133) for an active, non-axis cell: returns
 // the u2p code; sets flags and updates uu to follow the floored primitives.
-template <int MAG, int RAD>
+// With DEFER a cell that needs the entropy inversion or the b^2/rho floor
+// returns CP_DEFERRED before writing uu (pp may be changed) and is redone by
+// the caller with DEFER false.
+constexpr int CP_DEFERRED = 100;
+template <int MAG, int RAD, bool DEFER = false>
 KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
                                int &fl_entr2, int &fl_hd, int &fl_rad)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};
   fl_entr = 0;
   fl_entr2 = 0;
-  int r = u2p<MAG, RAD>(P, uu, pp, g, corrected, fixups);
+  int r = u2p<MAG, RAD, DEFER>(P, uu, pp, g, corrected, fixups);
+  if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED;
   if (corrected[0] == 1) fl_entr = 1;
   if (corrected[2] == 1) fl_entr2 = 1;
-  floors_mhd<MAG, RAD>(P, pp, g);
+  if (floors_mhd<MAG, RAD, DEFER>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;
   if (RAD) floors_rad<MAG, RAD>(P, pp, g);
   p2u<MAG, RAD>(P, pp, uu, g);
   fl_hd = (fixups[0] > 0) ? 1 : 0;

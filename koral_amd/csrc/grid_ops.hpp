@@ -672,6 +672,10 @@ struct OpUpdate {
 // u2p sweep (calc_u2p finite.c:502-561 / calc_primitives u2p.c:13-133);
 // counts {hd fixup requests, rad fixup requests, entropy inversions}
 // ---------------------------------------------------------------------------
+// calc_primitives per interior cell.  A cell whose hot inversion fails
+// (entropy fallback) or whose state needs the b^2/rho floor is appended to
+// list (length *cnt, cleared before) and left unwritten for OpU2PFloor; the
+// common path then carries one Newton inversion instead of four.
 template <int MAG, int RAD>
 struct OpU2P {
   static constexpr int kind = K_U2P;
@@ -680,8 +684,12 @@ struct OpU2P {
   DGrid G;
   double *u, *p;
   signed char *fl;
+  unsigned *list, *cnt;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
-  KD_INL Cnt3 operator()(long i) const
+  KD_INL Cnt3 operator()(long i) const { return cell<true>(i, list, cnt); }
+  // the full calc_primitives (DEFER false) or the deferring one on cell i
+  template <bool DEFER>
+  KD_INL Cnt3 cell(long i, unsigned *lst, unsigned *len) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
@@ -694,7 +702,11 @@ struct OpU2P {
     Geom g;
     geom_cell(G, ix, iy, g);
     int fe, fe2, fh, fr;
-    calc_primitives<MAG, RAD>(P, uu, pp, g, fe, fe2, fh, fr);
+    const int r = calc_primitives<MAG, RAD, DEFER>(P, uu, pp, g, fe, fe2, fh, fr);
+    if (DEFER) {
+      append_index(r == CP_DEFERRED, (unsigned)i, lst, len);
+      if (r == CP_DEFERRED) return Cnt3{{0, 0, 0}};
+    }
     store_p<MAG, RAD>(p, G.ncell, c, pp);
     store_p<MAG, RAD>(u, G.ncell, c, uu);
     const long n = G.ncell;
@@ -703,6 +715,27 @@ struct OpU2P {
     fl[F_HDFIXUP * n + c] = (signed char)fh;
     fl[F_RADFIXUP * n + c] = (signed char)fr;
     return Cnt3{{fh != 0, fr != 0, fe != 0}};
+  }
+};
+
+// the cells OpU2P left, with the full calc_primitives; FLOOR_THREADS
+// threads stride over the list, whose length is known only on the device
+template <int MAG, int RAD>
+struct OpU2PFloor {
+  static constexpr int kind = K_U2P;
+  static constexpr int waves_per_simd = 2;
+  OpU2P<MAG, RAD> op;
+  KD_HD long size() const { return FLOOR_THREADS; }
+  KD_INL Cnt3 operator()(long t) const
+  {
+    Cnt3 sum{{0, 0, 0}};
+    const long m = (long)*op.cnt;
+    for (long j = t; j < m; j += FLOOR_THREADS) {
+      const Cnt3 c = op.template cell<false>((long)op.list[j], nullptr, nullptr);
+#pragma unroll
+      for (int k = 0; k < 3; k++) sum.v[k] += c.v[k];
+    }
+    return sum;
   }
 };
 
@@ -908,7 +941,8 @@ struct GridBufs {
   double *ws, *flb[3], *emf;
   double *fs[3];      // face states of the sweep of each direction, 2 NV x face_states_cap(G)
   unsigned *flist[3];  // faces of each sweep left to the b^2/rho floor pass, face_states_cap(G)
-  unsigned *fcnt;      // their counts [3]
+  unsigned *fcnt;      // their counts [3], then the count of ulist
+  unsigned *ulist;     // cells left to the full calc_primitives pass, nx ny nz
   double *pn, *un;
   int *mark;
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
@@ -1097,7 +1131,10 @@ void u2p(Ex &ex, const GridBufs &s)
 {
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
-    ex.count(OpU2P<T::MAG, T::RAD>{s.P, s.G, s.u, s.p, s.fl}, s.counters + 3);
+    ex.each(OpZeroU32{s.fcnt + 3, 1});
+    const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3};
+    ex.count(op, s.counters + 3);
+    ex.count(OpU2PFloor<T::MAG, T::RAD>{op}, s.counters + 3);
   });
   fixup(ex, s, F_HDFIXUP);
   if (s.P.rad) fixup(ex, s, F_RADFIXUP);
