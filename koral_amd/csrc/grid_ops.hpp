@@ -17,6 +17,7 @@
 // The library's executor launches one HIP kernel per call on the grid's
 // stream (k_grid.hip).
 #pragma once
+#include <initializer_list>
 #include <type_traits>
 #include "dev_grid.hpp"
 
@@ -25,7 +26,7 @@ namespace kd {
 // Operator kinds (per-kind launch timing and names in the C-ABI)
 enum OpKind {
   K_P2U, K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_FLUX, K_EMF, K_CT, K_UPDATE,
-  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_RECON, K_FLUX_FLOOR, K_NKINDS
+  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_RECON, K_FLUX_FLOOR, K_U2P_FLOOR, K_NKINDS
 };
 
 struct Box {
@@ -722,7 +723,7 @@ struct OpU2P {
 // threads stride over the list, whose length is known only on the device
 template <int MAG, int RAD>
 struct OpU2PFloor {
-  static constexpr int kind = K_U2P;
+  static constexpr int kind = K_U2P_FLOOR;
   static constexpr int waves_per_simd = 2;
   OpU2P<MAG, RAD> op;
   KD_HD long size() const { return FLOOR_THREADS; }
@@ -879,7 +880,6 @@ struct OpImplicit {
   }
 };
 
-// RK2IMEX linear combination over the domain: out = a x + b y (+ c z)
 // A coefficient of the step algebra as the expression of ko.c:444-570 it
 // stands for, evaluated on the device from the clock's dt with the same
 // operations the host would use: c, c / (dt * gamma), c / dt, dt * c, dt / c.
@@ -898,33 +898,70 @@ struct Coef {
   }
 };
 
-struct OpComb {
+// RK2IMEX stage algebra (ko.c:444-570): up to three statements out = a x
+// [+ b y [+ c z]], evaluated left to right, per variable of one interior
+// cell, each term an array or the result of an earlier statement of the
+// same launch.  A statement's arithmetic does not depend on what it is
+// fused with, so fusing gives the same bits as separate launches; each
+// input array is read once per launch.
+struct CombTerm {
+  Coef c;
+  const double *x;  // the array, or null for the result of statement prev
+  int prev;
+};
+struct CombStmt {
+  double *out;  // null: the result only feeds later statements
+  int nt;
+  CombTerm t[3];
+};
+struct OpCombSeq {
   static constexpr int kind = K_COMB;
   DGrid G;
   int nv;
-  double *out;
-  Coef ca, cb, cc;
-  const double *x, *y, *z;
+  int ns;
+  CombStmt st[3];
   const double *clk;
   double gamma;
-  KD_HD long size() const { return (long)G.nx * G.ny * G.nz * nv; }
+  // one interior cell per item, all its variables
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   KD_INL void operator()(long i) const
   {
     const double dt = clk[1];
-    const double a = ca.eval(dt, gamma), b = cb.eval(dt, gamma), c = cc.eval(dt, gamma);
-    long nd = (long)G.nx * G.ny * G.nz;
-    int iv = (int)(i / nd);
-    long r = i % nd;
-    int ix = (int)(r % G.nx), iy = (int)((r / G.nx) % G.ny), iz = (int)(r / ((long)G.nx * G.ny));
-    long k = (long)iv * G.ncell + cidx(G, ix, iy, iz);
-    double v;
-    if (z)
-      v = a * x[k] + b * y[k] + c * z[k];
-    else if (y)
-      v = a * x[k] + b * y[k];
-    else
-      v = a * x[k];
-    out[k] = v;
+    const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
+    const long c = cidx(G, ix, iy, iz);
+    double coef[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) coef[q][j] = st[q].t[j].c.eval(dt, gamma);
+    for (int iv = 0; iv < nv; iv++) {
+      const long k = (long)iv * G.ncell + c;
+      double r0 = 0., r1 = 0., r2 = 0.;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        if (q >= ns) break;
+        const CombStmt &S = st[q];
+        double v[3] = {0., 0., 0.};
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          if (j < S.nt) {
+            const int pq = S.t[j].prev;
+            v[j] = S.t[j].x ? S.t[j].x[k] : (pq == 0 ? r0 : (pq == 1 ? r1 : r2));
+          }
+        const double a = coef[q][0], b = coef[q][1], cc = coef[q][2];
+        double w;
+        if (S.nt == 3)
+          w = a * v[0] + b * v[1] + cc * v[2];
+        else if (S.nt == 2)
+          w = a * v[0] + b * v[1];
+        else
+          w = a * v[0];
+        if (q == 0) r0 = w;
+        if (q == 1) r1 = w;
+        if (q == 2) r2 = w;
+        if (S.out) S.out[k] = w;
+      }
+    }
   }
 };
 
@@ -1225,7 +1262,10 @@ void comb(Ex &ex, const GridBufs &s, double gamma, double *out, Coef a, const do
           Coef b = {Coef::CONST, 0.}, const double *y = nullptr, Coef c = {Coef::CONST, 0.},
           const double *z = nullptr)
 {
-  ex.each(OpComb{s.G, s.P.nv, out, a, b, c, x, y, z, s.clock, gamma});
+  const int nt = z ? 3 : (y ? 2 : 1);
+  OpCombSeq op{s.G, s.P.nv, 1, {}, s.clock, gamma};
+  op.st[0] = CombStmt{out, nt, {CombTerm{a, x, 0}, CombTerm{b, y, 0}, CombTerm{c, z, 0}}};
+  ex.each(op);
 }
 
 // body of one RK2IMEX step, ko.c:444-570, at the clock's (t, dt); the
@@ -1237,27 +1277,41 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   const Coef one{Coef::CONST, 1.}, idtg{Coef::C_OVER_DTG, 1.}, midtg{Coef::C_OVER_DTG, -1.};
   const Coef idt{Coef::C_OVER_DT, 1.}, midt{Coef::C_OVER_DT, -1.}, dt1{Coef::DT_TIMES_C, 1.};
   const Coef dtc{Coef::DT_TIMES_C, 1. - 2. * gamma}, dth{Coef::DT_OVER_C, 2.};
+  // fused combinations: statements of one launch run in the order listed,
+  // R(c, q) is the result of statement q of the same launch
+  auto A = [](Coef c, const double *x) { return CombTerm{c, x, 0}; };
+  auto R = [](Coef c, int q) { return CombTerm{c, nullptr, q}; };
+  const CombTerm none{{Coef::CONST, 0.}, nullptr, 0};
+  auto seq = [&](std::initializer_list<CombStmt> l) {
+    OpCombSeq op{s.G, s.P.nv, (int)l.size(), {}, s.clock, gamma};
+    int q = 0;
+    for (const CombStmt &st : l) op.st[q++] = st;
+    ex.each(op);
+  };
   comb(ex, s, gamma, s.ut0, one, s.u);
   implicit_op(ex, s, gamma);
-  comb(ex, s, gamma, s.drt1, idtg, s.u, midtg, s.ut0);
-  comb(ex, s, gamma, s.ut1, one, s.u);
+  // drt1 = (u - ut0) / (gamma dt); ut1 = u
+  seq({CombStmt{s.drt1, 2, {A(idtg, s.u), A(midtg, s.ut0), none}}, CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
   u2p(ex, s);
   do_correct(ex, s);
   explicit_op(ex, s);
-  comb(ex, s, gamma, s.dut1, idt, s.u, midt, s.ut1);
-  comb(ex, s, gamma, s.u, one, s.ut0, dt1, s.dut1, dtc, s.drt1);
-  comb(ex, s, gamma, s.uforget, one, s.u);
+  // dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
+  seq({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+       CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
+       CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
   u2p(ex, s);
   do_correct(ex, s);
   implicit_op(ex, s, gamma);
-  comb(ex, s, gamma, s.drt2, idtg, s.u, midtg, s.uforget);
-  comb(ex, s, gamma, s.ut1, one, s.u);  // ut2
+  // drt2 = (u - uforget) / (gamma dt); ut2 = u
+  seq({CombStmt{s.drt2, 2, {A(idtg, s.u), A(midtg, s.uforget), none}},
+       CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
   u2p(ex, s);
   do_correct(ex, s);
   explicit_op(ex, s);
-  comb(ex, s, gamma, s.dut2, idt, s.u, midt, s.ut1);
-  comb(ex, s, gamma, s.u, one, s.ut0, dth, s.dut1, dth, s.dut2);
-  comb(ex, s, gamma, s.u, one, s.u, dth, s.drt1, dth, s.drt2);
+  // dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
+  seq({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+       CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
+       CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
   u2p(ex, s);
   do_correct(ex, s);
 }
