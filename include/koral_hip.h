@@ -81,6 +81,16 @@ int koral_hip_u2p_solver_batch(const koral_params *P, long n, const double *uu, 
                                const double *g20, const double *G20, int etype, int *ret,
                                void *stream);
 
+/* u2p_solver_W from a learned W start (no reference counterpart): W0[i] > 0
+   starts the Newton iteration in W instead of the start computed from pp
+   (u2p.c:2094-2112), under the unchanged tolerance; a start that fails is
+   retried from the plain start.  iters[i] (may be NULL) counts the Newton
+   iterations of both tries, guessed[i] (may be NULL) is 1 when the learned
+   start solved the point.  W0 NULL: the plain solver, with iteration counts. */
+int koral_hip_u2p_solver_batch_from(const koral_params *P, long n, const double *uu, double *pp,
+                                    const double *g20, const double *G20, int etype, const double *W0, int *ret,
+                                    int *iters, int *guessed, void *stream);
+
 /* u2p_rad (u2prad.c:485-511): M1 inversion of R^t_mu; corr[i] = corrected */
 int koral_hip_u2p_rad_batch(const koral_params *P, long n, const double *uu, double *pp,
                             const double *g20, const double *G20, int *ret, int *corr,
@@ -106,7 +116,8 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
 
    A small network (11 -> 128 -> 128 -> 128 -> 4, GELU) maps the local state
    of a cell to corrections of the Newton start.  kind 0 (gas) corrects
-   (u, u~^1..3), kind 1 (radiation) corrects (E, F~^1..3).  Parameters are
+   (u, u~^1..3), kind 1 (radiation) corrects (E, F~^1..3), kind 2 predicts the
+   W start of u2p (below).  Parameters are
    float32 [koral_hip_mlp_nparam()] in the order W1[128][11], b1[128],
    W2[128][128], b2, W3[128][128], b3, W4[4][128], b4 (W[out][in]); norm is
    float32 [30]: feature mean[11], std[11], target mean[4], std[4].
@@ -127,6 +138,21 @@ int koral_hip_mlp_features(const koral_params *P, const koral_hip_mlp *m, long n
                            const double *dt, float *feat, void *stream);
 int koral_hip_mlp_targets(int kind, long n, int nv, const double *pp0, const double *pp1, float *targ,
                           void *stream);
+/* The W network (kind 2; no reference counterpart): a start for the W
+   inversion of u2p from the conserved state uu and the guess pp.  Features:
+   log10(u/rho), log10(b^2/rho + 1e-30), u~^i sqrt(g_ii) (3), log(D/D0),
+   asinh((U_1 - U0_1)/|U0_1|), asinh((U_k - U0_k)/(|U0_0| + |U0_1|)) for
+   k = 2..4, log10(W0/rho), where U0 = p2u(pp) and W0 = (rho + Gamma u)
+   gamma^2 of pp.  Targets: log(W1/W0) between the solution pp1 and the guess
+   pp0, and three zeros.  The guess is W0 exp(y0) (0, the plain start, where
+   not finite or not positive). */
+int koral_hip_mlp_u2p_features(const koral_params *P, const koral_hip_mlp *m, long n, const double *uu,
+                               const double *pp, const double *g20, const double *G20, float *feat,
+                               void *stream);
+int koral_hip_mlp_u2p_targets(const koral_params *P, long n, const double *pp0, const double *pp1,
+                              const double *g20, const double *G20, float *targ, void *stream);
+int koral_hip_mlp_u2p_guess(const koral_params *P, const koral_hip_mlp *m, long n, const double *pp,
+                            const double *g20, const double *G20, const float *Y, double *W0, void *stream);
 /* FP32 forward pass of a training batch, Y [4][B] */
 int koral_hip_mlp_forward(koral_hip_mlp *m, const float *X, int B, float *Y, void *stream);
 /* loss (host, may be NULL; synchronises) and gradient (device [nparam], may be NULL) of the
@@ -157,6 +183,10 @@ int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, cons
 int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, double *pp, const double *g20,
                                   const double *G20, const double *dt, double global_time, const double *gxg,
                                   const double *gxr, int maxiter, int *flags, int *info, void *stream);
+/* koral_hip_u2p_solver_batch_from with the W start of a kind-2 network */
+int koral_hip_u2p_solver_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, const double *uu,
+                                       double *pp, const double *g20, const double *G20, int etype, int *ret,
+                                       int *iters, int *guessed, void *stream);
 int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
                                      double *uu, double *pp, const double *g20, const double *G20,
                                      const double *dt, double global_time, int maxiter, int *flags, int *info,

@@ -298,8 +298,12 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
   return (W * W * W * (W + 2. * Bsq) - QdotBsq * (2. * W + Bsq)) <= W * W * (Qtsq - Bsq * Bsq);
 }
 
+// Wstart > 0 replaces the W start computed from pp (the learned W guess,
+// which has no reference counterpart); iters, when given, receives the
+// Newton iterations made.
 template <int MAG, int RAD>
-KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype)
+KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype,
+                        double Wstart = 0., int *iters = nullptr)
 {
   double gdetu = g.gdet;
   double alpha = sqrt(-1. / g.G[0][0]);
@@ -344,9 +348,10 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   U2PCons c{Qn, Qtsq, D, QdotBsq, Bsq, Sc};
 
   double gamma2 = 1. + qsq_of(pp[2], pp[3], pp[4], g);
-  double W = (pp[0] + P.gamma * pp[1]) * gamma2;
+  double W = Wstart > 0. ? Wstart : (pp[0] + P.gamma * pp[1]) * gamma2;
   double f0, dfdW, err;
   int i_increase = 0;
+  if (iters) *iters = 0;
   for (;;) {
     f0 = dfdW = 0.;
     f_u2p(P, etype, W - D, c, f0, dfdW, err);
@@ -363,6 +368,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   do {
     Wprev = W;
     iter++;
+    if (iters) *iters = iter;
     f_u2p(P, etype, W - D, c, f0, dfdW, err);
     if (err < P.u2pconv) break;
     if (dfdW == 0.) {

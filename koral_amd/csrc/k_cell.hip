@@ -69,6 +69,45 @@ __global__ void __launch_bounds__(256) k_u2p(koral_params P, long n, double *__r
   flags[F_RADFIXUP * n + i] = fr;
 }
 
+// the W inversion from a learned W start (W0[i] > 0), retried from the
+// plain start when that fails; total Newton iterations and whether the
+// learned start solved the point
+template <int MAG, int RAD>
+__global__ void __launch_bounds__(256) k_u2p_solver_from(koral_params P, long n, const double *__restrict__ uu,
+                                                         double *__restrict__ pp, const double *__restrict__ g20,
+                                                         const double *__restrict__ G20, int etype,
+                                                         const double *__restrict__ W0, int *__restrict__ ret,
+                                                         int *__restrict__ iters, int *__restrict__ guessed)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Geom g;
+  load_point<MAG, RAD>(g20, G20, n, i, g);
+  double p[NV], u[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    p[v] = pp[v * n + i];
+    u[v] = uu[v * n + i];
+  }
+  int it = 0, it2 = 0, r = -1;
+  bool gd = false;
+  const double w0 = W0 ? W0[i] : 0.;
+  if (w0 > 0.) {
+    r = u2p_solver_W<MAG, RAD>(P, u, p, g, etype, w0, &it);
+    gd = r >= 0;
+  }
+  if (!gd) {
+    r = u2p_solver_W<MAG, RAD>(P, u, p, g, etype, 0., &it2);
+    it += it2;
+  }
+  ret[i] = r;
+  if (iters) iters[i] = it;
+  if (guessed) guessed[i] = gd ? 1 : 0;
+#pragma unroll
+  for (int v = 0; v < NV; v++) pp[v * n + i] = p[v];
+}
+
 template <int MAG, int RAD>
 __global__ void __launch_bounds__(256) k_u2p_solver(koral_params P, long n, const double *__restrict__ uu,
                                                     double *__restrict__ pp, const double *__restrict__ g20,
@@ -314,6 +353,43 @@ int koral_hip_u2p_solver_batch(const koral_params *P, long n, const double *uu, 
   hipStream_t s = (hipStream_t)stream;
   KORAL_DISPATCH(P, k_u2p_solver, <<<grid_for(n, 256), 256, 0, s>>>(*P, n, uu, pp, g20, G20, etype, ret));
   return check_launch("koral_hip_u2p_solver_batch");
+}
+
+int koral_hip_u2p_solver_batch_from(const koral_params *P, long n, const double *uu, double *pp,
+                                    const double *g20, const double *G20, int etype, const double *W0, int *ret,
+                                    int *iters, int *guessed, void *stream)
+{
+  if (check_params(P) || n < 0 || !ret || (etype != 0 && etype != 1))
+    return set_error(KORAL_E_ARG, "u2p_solver_batch_from: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  KORAL_DISPATCH(P, k_u2p_solver_from,
+                 <<<grid_for(n, 256), 256, 0, s>>>(*P, n, uu, pp, g20, G20, etype, W0, ret, iters, guessed));
+  return check_launch("koral_hip_u2p_solver_batch_from");
+}
+
+int koral_hip_u2p_solver_batch_learned(const koral_params *P, koral_hip_mlp *m, long n, const double *uu,
+                                       double *pp, const double *g20, const double *G20, int etype, int *ret,
+                                       int *iters, int *guessed, void *stream)
+{
+  if (check_params(P) || n < 0 || !m || koral_hip_mlp_kind(m) != 2)
+    return set_error(KORAL_E_ARG, "u2p_solver_batch_learned: bad arguments");
+  if (n == 0) return 0;
+  if (n >= (1L << 31)) return set_error(KORAL_E_ARG, "u2p batch larger than 2^31 points");
+  hipStream_t s = (hipStream_t)stream;
+  ImpSet *set = nullptr;
+  int rc = acquire_set(n, s, &set);
+  if (rc) return rc;
+  rc = learned_buffers(set);
+  // the W guess lands in the first n doubles of the gas-guess buffer
+  if (!rc) rc = koral_hip_mlp_u2p_features(P, m, n, uu, pp, g20, G20, set->feat, stream);
+  if (!rc) rc = koral_hip_mlp_infer(m, set->feat, n, set->y, stream);
+  if (!rc) rc = koral_hip_mlp_u2p_guess(P, m, n, pp, g20, G20, set->y, set->gxg, stream);
+  if (!rc) rc = koral_hip_u2p_solver_batch_from(P, n, uu, pp, g20, G20, etype, set->gxg, ret, iters, guessed, stream);
+  int rc2 = release_set(set, s);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  return check_launch("koral_hip_u2p_solver_batch_learned");
 }
 
 int koral_hip_u2p_rad_batch(const koral_params *P, long n, const double *uu, double *pp, const double *g20,

@@ -1,8 +1,10 @@
-// Learned initial guess for the implicit radiation solve: a small MLP
+// Learned initial guesses for the implicit radiation solve and the u2p W
+// inversion: a small MLP
 // (11 -> 128 -> 128 -> 128 -> 4, GELU in tanh form) that maps the
 // dimensionless local state of a cell to corrections of the plain Newton
 // start.  A gas network corrects (u, u~^1..3), a radiation network
-// (E, F~^1..3); the solve picks one per cell.  Training is FP32 (forward,
+// (E, F~^1..3); the solve picks one per cell.  A W network (kind 2)
+// predicts the start of the W Newton iteration of u2p.  Training is FP32 (forward,
 // backward and Adam kernels below, deterministic reductions); inference runs
 // the weights in BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
 // accumulation).
@@ -89,6 +91,101 @@ __global__ void __launch_bounds__(256) k_targets(long n, int base, const double 
 #pragma unroll
   for (int i = 1; i < NOUT; i++)
     t[i * n + c] = (float)asinh((p1[(base + i) * n + c] - p0[(base + i) * n + c]) / VSCALE);
+}
+
+// The W network (kind 2): features of a point of the u2p W inversion from
+// its conserved state uu and the guess pp (the primitives the inversion
+// starts from), and the target log(W1 / W0) between the solution pp1 and
+// the guess pp0.  W = (rho + Gamma u) gamma^2, the start u2p.c:2094-2112
+// computes from a primitive state.
+template <int MAG>
+__device__ __forceinline__ double plain_W(const koral_params &P, const double *p, const Geom &g)
+{
+  return (p[0] + P.gamma * p[1]) * (1. + qsq_of(p[2], p[3], p[4], g));
+}
+
+template <int MAG>
+__device__ __forceinline__ void load_mhd(const double *__restrict__ a, long n, long c, double *x)
+{
+  constexpr int NVM = 6 + 3 * MAG;
+#pragma unroll
+  for (int k = 0; k < 9; k++) x[k] = k < NVM ? a[k * n + c] : 0.;
+}
+
+template <int MAG>
+__global__ void __launch_bounds__(256) k_u2p_features(koral_params P, long n, const double *__restrict__ uu,
+                                                      const double *__restrict__ pp, const double *__restrict__ g20,
+                                                      const double *__restrict__ G20, const float *__restrict__ norm,
+                                                      float *__restrict__ feat)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Geom g;
+  load_geom(g, [&](int k) { return g20[k * n + c]; }, [&](int k) { return G20[k * n + c]; });
+  double p[9], U[9], U0[9];
+  load_mhd<MAG>(pp, n, c, p);
+  load_mhd<MAG>(uu, n, c, U);
+  p2u_mhd<MAG, 0>(P, p, U0, g);
+  const double rho = p[0], u = p[1];
+  double bsq = 0.;
+  if (MAG) {
+    double ucon[4], ucov[4], bcon[4], bcov[4];
+    relr_to_4vel(p[2], p[3], p[4], ucon, ucov, g);
+    bcon4(p[6], p[7], p[8], ucon, ucov, bcon);
+    lower(bcon, bcov, g);
+    bsq = dot4(bcon, bcov);
+  }
+  double f[NIN];
+  f[0] = log10(u / rho);
+  f[1] = log10(bsq / rho + 1e-30);
+#pragma unroll
+  for (int i = 0; i < 3; i++) f[2 + i] = p[2 + i] * sqrt(g.g[1 + i][1 + i]);
+  f[5] = log(U[0] / U0[0]);
+  f[6] = asinh((U[1] - U0[1]) / fabs(U0[1]));
+  const double s2 = fabs(U0[0]) + fabs(U0[1]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) f[7 + i] = asinh((U[2 + i] - U0[2 + i]) / s2);
+  f[10] = log10(plain_W<MAG>(P, p, g) / rho);
+#pragma unroll
+  for (int k = 0; k < NIN; k++) {
+    float v = (float)f[k];
+    if (norm) v = (v - norm[k]) / norm[NIN + k];
+    feat[k * n + c] = v;
+  }
+}
+
+template <int MAG>
+__global__ void __launch_bounds__(256) k_u2p_targets(koral_params P, long n, const double *__restrict__ p0,
+                                                     const double *__restrict__ p1, const double *__restrict__ g20,
+                                                     const double *__restrict__ G20, float *__restrict__ t)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Geom g;
+  load_geom(g, [&](int k) { return g20[k * n + c]; }, [&](int k) { return G20[k * n + c]; });
+  double a[9], b[9];
+  load_mhd<MAG>(p0, n, c, a);
+  load_mhd<MAG>(p1, n, c, b);
+  t[c] = (float)log(plain_W<MAG>(P, b, g) / plain_W<MAG>(P, a, g));
+#pragma unroll
+  for (int i = 1; i < NOUT; i++) t[i * n + c] = 0.f;
+}
+
+template <int MAG>
+__global__ void __launch_bounds__(256) k_u2p_guess(koral_params P, long n, const double *__restrict__ pp,
+                                                   const double *__restrict__ g20, const double *__restrict__ G20,
+                                                   const float *__restrict__ y, const float *__restrict__ norm,
+                                                   double *__restrict__ W0)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Geom g;
+  load_geom(g, [&](int k) { return g20[k * n + c]; }, [&](int k) { return G20[k * n + c]; });
+  double p[9];
+  load_mhd<MAG>(pp, n, c, p);
+  const double t = (double)y[c] * norm[2 * NIN + NOUT] + norm[2 * NIN];
+  const double w = plain_W<MAG>(P, p, g) * exp(t);
+  W0[c] = (isfinite(w) && w > 0.) ? w : 0.;
 }
 
 // ---------------------------------------------------------------------------
@@ -431,7 +528,7 @@ int koral_hip_mlp_nparam(void) { return NPARAM; }
 
 int koral_hip_mlp_create(int kind, koral_hip_mlp **out)
 {
-  if (!out || kind < 0 || kind > 1) return set_error(KORAL_E_ARG, "mlp_create: bad arguments");
+  if (!out || kind < 0 || kind > 2) return set_error(KORAL_E_ARG, "mlp_create: bad arguments");
   auto *m = new koral_hip_mlp();
   m->kind = kind;
   int rc = 0;
@@ -510,6 +607,49 @@ int koral_hip_mlp_targets(int kind, long n, int nv, const double *pp0, const dou
   if (n == 0) return 0;
   k_targets<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, kind == 0 ? 1 : nv - 4, pp0, pp1, targ);
   return check_launch("mlp_targets");
+}
+
+int koral_hip_mlp_u2p_features(const koral_params *P, const koral_hip_mlp *m, long n, const double *uu,
+                               const double *pp, const double *g20, const double *G20, float *feat, void *stream)
+{
+  if (check_params(P) || n < 0 || !uu || !pp || !g20 || !G20 || !feat || (m && m->kind != 2))
+    return set_error(KORAL_E_ARG, "mlp_u2p_features: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const float *norm = m ? m->norm : nullptr;
+  if (P->mag)
+    k_u2p_features<1><<<grid1(n), 256, 0, s>>>(*P, n, uu, pp, g20, G20, norm, feat);
+  else
+    k_u2p_features<0><<<grid1(n), 256, 0, s>>>(*P, n, uu, pp, g20, G20, norm, feat);
+  return check_launch("mlp_u2p_features");
+}
+
+int koral_hip_mlp_u2p_targets(const koral_params *P, long n, const double *pp0, const double *pp1,
+                              const double *g20, const double *G20, float *targ, void *stream)
+{
+  if (check_params(P) || n < 0 || !pp0 || !pp1 || !g20 || !G20 || !targ)
+    return set_error(KORAL_E_ARG, "mlp_u2p_targets: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (P->mag)
+    k_u2p_targets<1><<<grid1(n), 256, 0, s>>>(*P, n, pp0, pp1, g20, G20, targ);
+  else
+    k_u2p_targets<0><<<grid1(n), 256, 0, s>>>(*P, n, pp0, pp1, g20, G20, targ);
+  return check_launch("mlp_u2p_targets");
+}
+
+int koral_hip_mlp_u2p_guess(const koral_params *P, const koral_hip_mlp *m, long n, const double *pp,
+                            const double *g20, const double *G20, const float *Y, double *W0, void *stream)
+{
+  if (check_params(P) || n < 0 || !m || m->kind != 2 || !pp || !g20 || !G20 || !Y || !W0)
+    return set_error(KORAL_E_ARG, "mlp_u2p_guess: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (P->mag)
+    k_u2p_guess<1><<<grid1(n), 256, 0, s>>>(*P, n, pp, g20, G20, Y, m->norm, W0);
+  else
+    k_u2p_guess<0><<<grid1(n), 256, 0, s>>>(*P, n, pp, g20, G20, Y, m->norm, W0);
+  return check_launch("mlp_u2p_guess");
 }
 
 int koral_hip_mlp_forward(koral_hip_mlp *m, const float *X, int B, float *Y, void *stream)

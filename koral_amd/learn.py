@@ -25,7 +25,7 @@ from .cell import _ptr, _stream, _chk
 
 NIN, NH, NOUT = 11, 128, 4
 BATCH = 4096
-GAS, RAD = 0, 1
+GAS, RAD, WU2P = 0, 1, 2
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 
@@ -50,6 +50,11 @@ def _sig(L):
         "koral_hip_implicit_batch_learned": [vp, vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp,
                                              vp],
         "koral_hip_implicit_batch_from": [vp, l, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, i, vp, vp, vp],
+        "koral_hip_mlp_u2p_features": [vp, vp, l, vp, vp, vp, vp, vp, vp],
+        "koral_hip_mlp_u2p_targets": [vp, l, vp, vp, vp, vp, vp, vp],
+        "koral_hip_mlp_u2p_guess": [vp, vp, l, vp, vp, vp, vp, vp, vp],
+        "koral_hip_u2p_solver_batch_from": [vp, l, vp, vp, vp, vp, i, vp, vp, vp, vp, vp],
+        "koral_hip_u2p_solver_batch_learned": [vp, vp, l, vp, vp, vp, vp, i, vp, vp, vp, vp],
     }.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -75,7 +80,8 @@ def init_params(seed=0):
 
 
 class Net:
-    """One network on the device; kind GAS corrects (u, u~^i), RAD (E, F~^i)."""
+    """One network on the device; kind GAS corrects (u, u~^i), RAD (E, F~^i),
+    WU2P predicts the W start of the u2p inversion."""
 
     def __init__(self, kind=GAS, seed=0, theta=None, norm=None):
         self.L = _sig(lib())
@@ -183,6 +189,82 @@ class Net:
     def start(self, P, pp, dt):
         """Newton start [4][n] predicted for the cells of pp."""
         return self.guess(pp, self.infer(self.features(P, pp, dt)))
+
+
+    # ---- the W network ----
+    def u2p_features(self, P, uu, pp, g20, G20, standardise=True):
+        n = pp.shape[1]
+        F = torch.empty((NIN, n), dtype=torch.float32, device=pp.device)
+        check(self.L.koral_hip_mlp_u2p_features(ctypes.byref(P), self.h if standardise else None, n,
+                                                _ptr(_chk(uu)), _ptr(_chk(pp)), _ptr(_chk(g20)), _ptr(_chk(G20)),
+                                                _ptr(F), _stream()), "mlp_u2p_features")
+        return F
+
+    def u2p_start(self, P, uu, pp, g20, G20):
+        """W start [n] predicted for the points (0 where the plain start is kept)."""
+        n = pp.shape[1]
+        Y = self.infer(self.u2p_features(P, uu, pp, g20, G20))
+        W0 = torch.empty(n, dtype=torch.float64, device=pp.device)
+        check(self.L.koral_hip_mlp_u2p_guess(ctypes.byref(P), self.h, n, _ptr(_chk(pp)), _ptr(_chk(g20)),
+                                             _ptr(_chk(G20)), _ptr(_chk(Y, torch.float32)), _ptr(W0), _stream()),
+              "mlp_u2p_guess")
+        return W0
+
+
+def u2p_raw_features(P, uu, pp, g20, G20):
+    L = _sig(lib())
+    n = pp.shape[1]
+    F = torch.empty((NIN, n), dtype=torch.float32, device=pp.device)
+    check(L.koral_hip_mlp_u2p_features(ctypes.byref(P), None, n, _ptr(_chk(uu)), _ptr(_chk(pp)), _ptr(_chk(g20)),
+                                       _ptr(_chk(G20)), _ptr(F), _stream()), "mlp_u2p_features")
+    return F
+
+
+def u2p_raw_targets(P, pp0, pp1, g20, G20):
+    L = _sig(lib())
+    n = pp0.shape[1]
+    T = torch.empty((NOUT, n), dtype=torch.float32, device=pp0.device)
+    check(L.koral_hip_mlp_u2p_targets(ctypes.byref(P), n, _ptr(_chk(pp0)), _ptr(_chk(pp1)), _ptr(_chk(g20)),
+                                      _ptr(_chk(G20)), _ptr(T), _stream()), "mlp_u2p_targets")
+    return T
+
+
+def u2p_solver_from(P, uu, pp, g20, G20, W0=None, etype=0):
+    """koral_hip_u2p_solver_batch_from in place on pp: (ret, iters, guessed)."""
+    L = _sig(lib())
+    n = pp.shape[1]
+    ret = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    iters = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    guessed = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    check(L.koral_hip_u2p_solver_batch_from(ctypes.byref(P), n, _ptr(_chk(uu)), _ptr(_chk(pp)), _ptr(_chk(g20)),
+                                            _ptr(_chk(G20)), int(etype),
+                                            _ptr(_chk(W0)) if W0 is not None else None, _ptr(ret), _ptr(iters),
+                                            _ptr(guessed), _stream()), "u2p_solver_from")
+    return ret, iters, guessed
+
+
+def u2p_solver_learned(P, net, uu, pp, g20, G20, etype=0):
+    """koral_hip_u2p_solver_batch_learned in place on pp: (ret, iters, guessed)."""
+    n = pp.shape[1]
+    ret = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    iters = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    guessed = torch.zeros(n, dtype=torch.int32, device=pp.device)
+    check(net.L.koral_hip_u2p_solver_batch_learned(ctypes.byref(P), net.h, n, _ptr(_chk(uu)), _ptr(_chk(pp)),
+                                                   _ptr(_chk(g20)), _ptr(_chk(G20)), int(etype), _ptr(ret),
+                                                   _ptr(iters), _ptr(guessed), _stream()), "u2p_solver_learned")
+    return ret, iters, guessed
+
+
+def u2p_training_pairs(P, uu, pp0, g20, G20):
+    """Raw features [11][m] and W targets [4][m] from the plain solver: the
+    points it inverts from the guess pp0 (hot inversion) give (features of
+    (uu, pp0), log(W* / W0))."""
+    pp1 = pp0.clone()
+    ret, _, _ = u2p_solver_from(P, uu, pp1, g20, G20, None)
+    F = u2p_raw_features(P, uu, pp0, g20, G20)
+    T = u2p_raw_targets(P, pp0, pp1, g20, G20)
+    keep = (ret == 0) & torch.isfinite(F).all(0) & torch.isfinite(T).all(0)
+    return F[:, keep].contiguous(), T[:, keep].contiguous()
 
 
 def raw_targets(pp0, pp1, kind=GAS):

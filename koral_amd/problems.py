@@ -115,6 +115,26 @@ def random_mhd_states(P, n, seed=0, spin=0.9, gamma_max=5.0, beta_range=(0.3, 1e
     return pp, g20, G20
 
 
+def u2p_guess_states(P, n, seed=0):
+    """Points for the u2p W inversion from a stale guess: true magnetised
+    states (random_mhd_states), the conserved state is taken from them by
+    the caller, and the guess is the true state perturbed as a timestep
+    would leave it: rho and u scaled by exp(s z) and each velocity shifted
+    by s z (|u~| + 0.1), with s log-uniform in [1e-4, 0.3] per point.
+    Returns (pp_true, pp_guess, g20, G20)."""
+    rng = np.random.default_rng(seed + 7919)
+    pp, g20, G20 = random_mhd_states(P, n, seed=seed, rad=False)
+    s = np.exp(rng.uniform(np.log(1e-4), np.log(0.3), n))
+    guess = pp.copy()
+    guess[:, 0] *= np.exp(s * rng.standard_normal(n))
+    guess[:, 1] *= np.exp(s * rng.standard_normal(n))
+    vmag = np.sqrt((pp[:, 2:5] ** 2).sum(1))
+    guess[:, 2:5] += (s * (vmag + 0.1))[:, None] * rng.standard_normal((n, 3))
+    n_ = 1.0 / (P.gamma - 1.)
+    guess[:, 5] = guess[:, 0] * np.log((P.gamma - 1.) * guess[:, 1]) * n_ - guess[:, 0] * np.log(guess[:, 0]) * (n_ + 1.0)
+    return pp, guess, g20, G20
+
+
 # ---------------------------------------------------------------------------
 # grid problems
 # ---------------------------------------------------------------------------

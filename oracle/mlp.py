@@ -134,3 +134,67 @@ def guess(pp0, t, base=1):
     g[:, 0] = pp0[:, base] * np.exp(t[:, 0])
     g[:, 1:4] = pp0[:, base + 1:base + 4] + VSCALE * np.sinh(t[:, 1:4])
     return g
+
+
+# ---- the W network of the u2p inversion ----
+def _metric(g20, G20):
+    """g_{mu nu}, g^{mu nu} [n, 4, 4] from the 20-entry rows (gg[i][j] at
+    5 i + j, struct geometry order)."""
+    n = g20.shape[0]
+    g = g20.reshape(n, 4, 5)[:, :, :4]
+    G = G20.reshape(n, 4, 5)[:, :, :4]
+    return g, G
+
+
+def _ucon(v, g, G):
+    """VELR u~^i -> u^mu (relele.c conv_vels, VELR -> VEL4)."""
+    qsq = np.einsum("ni,nj,nij->n", v, v, g[:, 1:, 1:])
+    alpha2 = -1. / G[:, 0, 0]
+    alpgam = np.sqrt(alpha2 * (1. + qsq))
+    u = np.empty((v.shape[0], 4))
+    u[:, 0] = np.sqrt((1. + qsq) / alpha2)
+    u[:, 1:] = v - alpgam[:, None] * G[:, 0, 1:]
+    return u, qsq
+
+
+def plain_W(P, pp, g20, G20):
+    """The W start of u2p_solver_W (u2p.c:2094-2112): (rho + Gamma u) gamma^2."""
+    g, G = _metric(g20, G20)
+    _, qsq = _ucon(pp[:, 2:5], g, G)
+    return (pp[:, 0] + P.gamma * pp[:, 1]) * (1. + qsq)
+
+
+def u2p_raw_features(P, uu, pp, g20, G20, U0):
+    """Unstandardised W-network features [n, 11] of conserved states uu and
+    guesses pp [n, nv]; U0 = p2u(pp) (the oracle's p2u)."""
+    g, G = _metric(g20, G20)
+    rho, u = pp[:, 0], pp[:, 1]
+    ucon, _ = _ucon(pp[:, 2:5], g, G)
+    ucov = np.einsum("nij,nj->ni", g, ucon)
+    bsq = np.zeros(pp.shape[0])
+    if P.mag:
+        B = pp[:, 6:9]
+        b0 = np.einsum("ni,ni->n", B, ucov[:, 1:])
+        bcon = np.empty_like(ucon)
+        bcon[:, 0] = b0
+        bcon[:, 1:] = (B + b0[:, None] * ucon[:, 1:]) / ucon[:, 0:1]
+        bsq = np.einsum("ni,nij,nj->n", bcon, g, bcon)
+    f = np.empty((pp.shape[0], NIN))
+    f[:, 0] = np.log10(u / rho)
+    f[:, 1] = np.log10(bsq / rho + 1e-30)
+    for i in range(3):
+        f[:, 2 + i] = pp[:, 2 + i] * np.sqrt(g[:, 1 + i, 1 + i])
+    f[:, 5] = np.log(uu[:, 0] / U0[:, 0])
+    f[:, 6] = np.arcsinh((uu[:, 1] - U0[:, 1]) / np.abs(U0[:, 1]))
+    s2 = np.abs(U0[:, 0]) + np.abs(U0[:, 1])
+    for i in range(3):
+        f[:, 7 + i] = np.arcsinh((uu[:, 2 + i] - U0[:, 2 + i]) / s2)
+    f[:, 10] = np.log10(plain_W(P, pp, g20, G20) / rho)
+    return f
+
+
+def u2p_targets(P, pp0, pp1, g20, G20):
+    """Raw W targets [n, 4]: log(W(pp1) / W(pp0)) and three zeros."""
+    t = np.zeros((pp0.shape[0], NOUT))
+    t[:, 0] = np.log(plain_W(P, pp1, g20, G20) / plain_W(P, pp0, g20, G20))
+    return t

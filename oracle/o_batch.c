@@ -71,6 +71,34 @@ void oracle_u2p_solver_batch(const oracle_params *P, long n, const double *uu, d
   }
 }
 
+/* The W inversion from a learned W start W0[i] (> 0; the plain start
+   otherwise) under the unchanged tolerance; a start that fails is retried
+   from the plain start, as the chain would run without it.  iters[i] counts
+   the Newton iterations of both tries, guessed[i] is 1 when the learned start
+   solved the point.  No reference counterpart. */
+void oracle_u2p_solver_batch_from(const oracle_params *P, long n, const double *uu, double *pp,
+                                  const double *g20, const double *G20, int etype, const double *W0, int *ret,
+                                  int *iters, int *guessed)
+{
+  int nv = P->nv;
+  for (long i = 0; i < n; i++) {
+    o_geom g;
+    o_fill_geom(&g, g20 + 20 * i, G20 + 20 * i);
+    int it = 0, it2 = 0, r = -1, gd = 0;
+    if (W0 && W0[i] > 0.) {
+      r = o_u2p_solver_W_from(P, uu + nv * i, pp + nv * i, &g, etype, W0[i], &it);
+      gd = r >= 0;
+    }
+    if (!gd) {
+      r = o_u2p_solver_W_from(P, uu + nv * i, pp + nv * i, &g, etype, 0., &it2);
+      it += it2;
+    }
+    ret[i] = r;
+    if (iters) iters[i] = it;
+    if (guessed) guessed[i] = gd;
+  }
+}
+
 void oracle_u2p_rad_batch(const oracle_params *P, long n, const double *uu, double *pp,
                           const double *g20, const double *G20, int *ret, int *corr)
 {

@@ -248,7 +248,18 @@ static int w_out_of_bounds(double W, double Bsq, double QdotBsq, double Qtsq)
   return (W * W * W * (W + 2. * Bsq) - QdotBsq * (2. * W + Bsq)) <= W * W * (Qtsq - Bsq * Bsq);
 }
 
+/* u2p_solver_W from a given W start (Wstart > 0; the learned W guess, no
+   reference counterpart) reporting its Newton iterations in *iters */
+int o_u2p_solver_W_from(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int etype,
+                        double Wstart, int *iters);
+
 int o_u2p_solver_W(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int etype)
+{
+  return o_u2p_solver_W_from(P, uu, pp, g, etype, 0., NULL);
+}
+
+int o_u2p_solver_W_from(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int etype,
+                        double Wstart, int *iters)
 {
   void (*fu)(const oracle_params *, double, const double *, double *, double *, double *) =
       (etype == O_U2P_ENTROPY) ? f_u2p_entropy : f_u2p_hot;
@@ -296,11 +307,12 @@ int o_u2p_solver_W(const oracle_params *P, const double *uu, double *pp, const o
   double utcon[4] = {0., pp[2], pp[3], pp[4]};
   double gamma2 = 1. + qsq_of(utcon, g);
   double gamma;
-  double W = (pp[0] + P->gamma * pp[1]) * gamma2;
+  double W = Wstart > 0. ? Wstart : (pp[0] + P->gamma * pp[1]) * gamma2;
 
   double cons[7] = {Qn, Qtsq, D, QdotBsq, Bsq, Sc, Qdotnp};
   double f0, dfdW, err;
   int i_increase = 0;
+  if (iters) *iters = 0;
   for (;;) {
     f0 = dfdW = 0.;
     fu(P, W - D, cons, &f0, &dfdW, &err);
@@ -318,6 +330,7 @@ int o_u2p_solver_W(const oracle_params *P, const double *uu, double *pp, const o
   do {
     Wprev = W;
     iter++;
+    if (iters) *iters = iter;
     fu(P, W - D, cons, &f0, &dfdW, &err);
     if (err < P->u2pconv) break;
     if (dfdW == 0.) {

@@ -122,6 +122,8 @@ int o_p2u_mhd(const oracle_params *P, const double *pp, double *uu, const o_geom
 int o_p2u_rad(const oracle_params *P, const double *pp, double *uu, const o_geom *g);
 double o_calc_Sfromu(const oracle_params *P, double rho, double u);
 int o_u2p_solver_W(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int etype);
+int o_u2p_solver_W_from(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int etype,
+                        double Wstart, int *iters);
 int o_u2p(const oracle_params *P, const double *uu0, double *pp, const o_geom *g,
           int corrected[3], int fixups[2]);
 int o_u2p_rad(const oracle_params *P, const double *uu, double *pp, const o_geom *g, int *corrected);
@@ -169,6 +171,9 @@ void oracle_u2p_batch(const oracle_params *P, long n, double *uu, double *pp,
                       const double *g20, const double *G20, int *ret, int *flags);
 void oracle_u2p_solver_batch(const oracle_params *P, long n, const double *uu, double *pp,
                              const double *g20, const double *G20, int etype, int *ret);
+void oracle_u2p_solver_batch_from(const oracle_params *P, long n, const double *uu, double *pp,
+                                  const double *g20, const double *G20, int etype, const double *W0, int *ret,
+                                  int *iters, int *guessed);
 void oracle_u2p_rad_batch(const oracle_params *P, long n, const double *uu, double *pp,
                           const double *g20, const double *G20, int *ret, int *corr);
 void oracle_floors_batch(const oracle_params *P, long n, double *pp,

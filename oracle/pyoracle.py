@@ -34,6 +34,8 @@ def lib():
         L.oracle_p2u_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp]
         L.oracle_u2p_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _ip, _ip]
         L.oracle_u2p_solver_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_int, _ip]
+        L.oracle_u2p_solver_batch_from.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_int, vp,
+                                                   _ip, _ip, _ip]
         L.oracle_u2p_rad_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _dp, _ip, _ip]
         L.oracle_floors_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, _ip]
         L.oracle_Gi_batch.argtypes = [vp, ctypes.c_long, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -90,6 +92,23 @@ def u2p_solver(P, uu, pp_guess, g20, G20, etype=0):
     lib().oracle_u2p_solver_batch(_P(P), n, _d(uu), _d(pp), _d(np.ascontiguousarray(g20)),
                                   _d(np.ascontiguousarray(G20)), etype, _i(ret))
     return pp, ret
+
+
+def u2p_solver_from(P, uu, pp_guess, g20, G20, W0=None, etype=0):
+    """The W inversion with a learned W start per point (None: the plain
+    start for all): (pp, ret, iters, guessed)."""
+    uu = np.ascontiguousarray(uu, dtype=np.float64)
+    pp = np.array(pp_guess, dtype=np.float64, order="C")
+    n = pp.shape[0]
+    ret = np.zeros(n, np.int32)
+    iters = np.zeros(n, np.int32)
+    guessed = np.zeros(n, np.int32)
+    w = None if W0 is None else np.ascontiguousarray(W0, dtype=np.float64)
+    lib().oracle_u2p_solver_batch_from(_P(P), n, _d(uu), _d(pp), _d(np.ascontiguousarray(g20)),
+                                       _d(np.ascontiguousarray(G20)), etype,
+                                       None if w is None else w.ctypes.data_as(ctypes.c_void_p),
+                                       _i(ret), _i(iters), _i(guessed))
+    return pp, ret, iters, guessed
 
 
 def u2p_rad(P, uu, pp_guess, g20, G20):

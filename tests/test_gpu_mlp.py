@@ -1,7 +1,8 @@
 """GPU checks of the learned-guess network kernels against the numpy
 reference (oracle/mlp.py): features, FP32 forward, gradient, Adam step, the
 BF16 matrix-core inference (which pins the operand lane layout), and the
-implicit solve with the trained guess against the plain solve."""
+implicit solve with the trained guess against the plain solve; the W
+network of the u2p inversion and the inversion from its starts."""
 import sys
 
 import numpy as np
@@ -247,3 +248,106 @@ def test_learned_solve_matches_oracle(dev, oracle):
         assert (info[:, 1] == 6).sum() > 0
         assert same.mean() > 0.99
         assert rel.max() < 1e-9
+
+
+# ---- the W network of the u2p inversion ----
+def _u2p_batch(dev, oracle, n, seed):
+    from koral_amd import params as KP, problems as PR, cell
+    P = KP.params_torus_mhd()
+    pt, pg, g20, G20 = PR.u2p_guess_states(P, n, seed=seed)
+    uu = oracle.p2u(P, pt, g20, G20)
+    return P, pt, pg, g20, G20, uu
+
+
+def test_u2p_features_targets_match_reference(dev, oracle):
+    """W-network feature and target kernels against the numpy restatement."""
+    import mlp as M
+    from koral_amd import cell
+    from koral_amd import learn as LN
+    P, pt, pg, g20, G20, uu = _u2p_batch(dev, oracle, 4096, 31)
+    U, Pg, Pt, g, G = (cell.to_dev(a) for a in (uu, pg, pt, g20, G20))
+    F = LN.u2p_raw_features(P, U, Pg, g, G).cpu().numpy().T
+    ref = M.u2p_raw_features(P, uu, pg, g20, G20, oracle.p2u(P, pg, g20, G20))
+    ferr = np.abs(F - ref) / (np.abs(ref) + 1e-3)
+    T = LN.u2p_raw_targets(P, Pg, Pt, g, G).cpu().numpy().T
+    tref = M.u2p_targets(P, pg, pt, g20, G20)
+    terr = np.abs(T - tref) / (np.abs(tref) + 1e-3)
+    _say("W features max rel %.2e, targets max rel %.2e" % (ferr.max(), terr.max()))
+    assert ferr.max() < 1e-5 and terr.max() < 1e-5
+
+
+def test_u2p_solver_from_matches_oracle(dev, oracle):
+    """The device W inversion from given W starts (good, poor and failing)
+    against the oracle: return codes, iteration counts and the learned-start
+    flag identical, primitives to the parity bar."""
+    import torch
+    import mlp as M
+    from koral_amd import cell
+    from koral_amd import learn as LN
+    P, pt, pg, g20, G20, uu = _u2p_batch(dev, oracle, 8192, 32)
+    rng = np.random.default_rng(5)
+    W = M.plain_W(P, pt, g20, G20) * np.exp(rng.normal(scale=0.01, size=len(pt)))
+    W[:1000] = M.plain_W(P, pg[:1000], g20[:1000], G20[:1000]) * 1e30  # starts that fail
+    W[1000:1500] = -1.                                                  # no start
+    U, g, G = cell.to_dev(uu), cell.to_dev(g20), cell.to_dev(G20)
+    for W0 in (None, W):
+        Pd = cell.to_dev(pg)
+        r, it, gd = LN.u2p_solver_from(P, U, Pd, g, G, None if W0 is None else torch.from_numpy(W0).to(dev))
+        po = cell.to_host(Pd)
+        pr, rr, ir, gr = oracle.u2p_solver_from(P, uu, pg, g20, G20, W0)
+        r, it, gd = r.cpu().numpy(), it.cpu().numpy(), gd.cpu().numpy()
+        ok = rr == 0
+        rel = np.abs(po[ok] - pr[ok]) / (np.abs(pr[ok]) + 1e-300)
+        _say("W start %s: solved %d/%d, learned start used %d, mean iters %.3f, max rel p %.2e" %
+             ("none" if W0 is None else "given", ok.sum(), len(ok), gd.sum(), it.mean(), rel.max()))
+        assert np.array_equal(r, rr) and np.array_equal(it, ir) and np.array_equal(gd, gr)
+        assert rel.max() < 1e-10
+
+
+def test_learned_u2p_guess(dev, oracle):
+    """Train the W network on pairs from the plain solver and invert a fresh
+    batch from its W starts: every point the plain solver inverts is
+    inverted, to the solver tolerance, in no more Newton iterations on
+    average; points whose learned start fails get the plain result.  (The
+    gain is small: the plain start from the previous primitives is already
+    within a few per cent of W and the Newton iteration converges
+    quadratically, while the network's start is good to about 1 %.)"""
+    import torch
+    from koral_amd import cell
+    from koral_amd import learn as LN
+    P, pt, pg, g20, G20, uu = _u2p_batch(dev, oracle, 1 << 18, 33)
+    U, Pg, g, G = (cell.to_dev(a) for a in (uu, pg, g20, G20))
+    F, T = LN.u2p_training_pairs(P, U, Pg, g, G)
+    net, hist = LN.train(P, F, T, LN.WU2P, schedule=[(1e-3, 8), (3e-4, 4), (1e-4, 2)])
+    _say("W network: %d pairs, loss %s" % (F.shape[1], ["%.4f" % h for h in hist]))
+    assert hist[-1] < 0.5 * hist[0]
+    P2, pt2, pg2, g202, G202, uu2 = _u2p_batch(dev, oracle, 1 << 16, 34)
+    U2, g2, G2 = cell.to_dev(uu2), cell.to_dev(g202), cell.to_dev(G202)
+    Pa = cell.to_dev(pg2)
+    ra, ia, _ = LN.u2p_solver_from(P, U2, Pa, g2, G2, None)
+    Pb = cell.to_dev(pg2)
+    rb, ib, gb = LN.u2p_solver_learned(P, net, U2, Pb, g2, G2)
+    ra, ia, rb, ib, gb = (x.cpu().numpy() for x in (ra, ia, rb, ib, gb))
+    pa, pb = cell.to_host(Pa), cell.to_host(Pb)
+    ok = ra == 0
+    # rho and u~^i relative to themselves; u relative to rho + u (u comes from
+    # W / gamma^2 - rho, which cancels for cold states)
+    scale = np.abs(pa[ok, :5]) + 1e-300
+    scale[:, 1] = pa[ok, 0] + pa[ok, 1]
+    rel = np.abs(pb[ok, :5] - pa[ok, :5]) / scale
+    # the inversion stops on a residual below 1e-10 or on |dW/W| < 1e-10 with
+    # a residual below 0.1, so converged answers may differ beyond 1e-10;
+    # the learned answer must be as close to the true state as the plain one
+    err_a = (np.abs(pa[ok, :5] - pt2[ok, :5]) / scale).max(1)
+    err_b = (np.abs(pb[ok, :5] - pt2[ok, :5]) / scale).max(1)
+    _say("plain: solved %.4f, mean iters %.3f; learned: solved %.4f (learned start %.4f), mean iters %.3f (x%.3f); "
+         "learned vs plain max rel %.2e median %.2e; error vs true state plain max %.2e learned max %.2e" %
+         (ok.mean(), ia.mean(), (rb == 0).mean(), gb.mean(), ib.mean(), ia.mean() / ib.mean(), rel.max(),
+          np.median(rel), err_a.max(), err_b.max()))
+    assert (rb[ok] == 0).all()
+    assert np.median(rel) < 1e-10
+    assert err_b.max() <= 2. * err_a.max() + 1e-12
+    back = gb == 0
+    assert np.array_equal(pb[back], pa[back])
+    assert gb.mean() > 0.9
+    assert ib.mean() <= ia.mean()
