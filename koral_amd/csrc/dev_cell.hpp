@@ -20,22 +20,28 @@ namespace kd {
 // across waves by arrival): the wave ballots its lanes, one lane reserves
 // the wave's entries with one atomic add.  Every active lane of the wave
 // calls it.
+// Returns the entry's position for the lanes with pred, else -1.
 #ifdef KD_HOST_CHECK
-inline void append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
+inline long append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
 {
-  if (pred) list[(*len)++] = v;
+  if (!pred) return -1;
+  list[*len] = v;
+  return (long)(*len)++;
 }
 #else
-__device__ __forceinline__ void append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
+__device__ __forceinline__ long append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
 {
   const unsigned long long m = __ballot(pred);
-  if (!m) return;
+  if (!m) return -1;
   const int lane = threadIdx.x & 63;
   const int leader = __ffsll((unsigned long long)m) - 1;
   unsigned base = 0;
   if (lane == leader) base = atomicAdd(len, (unsigned)__popcll(m));
   base = __shfl(base, leader);
-  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+  if (!pred) return -1;
+  const unsigned slot = base + __popcll(m & ((1ull << lane) - 1ull));
+  list[slot] = v;
+  return (long)slot;
 }
 #endif
 
@@ -617,11 +623,14 @@ KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 // ---------------------------------------------------------------------------
 // u2p, u2p.c:176-485 (type 0)
 // With DEFER a state whose hot inversion fails is left for a later pass:
-// the function returns U2P_DEFERRED at once (pp then holds a failed
-// iterate) and the caller redoes the state with DEFER false, so the entropy
-// inversion is not inlined into the common path.
-constexpr int U2P_DEFERRED = 1;
-template <int MAG, int RAD, bool DEFER = false>
+// the function returns U2P_DEFERRED at once and the caller redoes the state
+// with DEFER false, so the entropy inversion is not inlined into the common
+// path.  A failed hot inversion leaves pp as it was unless it failed with
+// -105 (negative density after the update), so such a redo may start at the
+// entropy inversion (SKIP_HOT) and reach the same state; U2P_DEFERRED_105
+// marks the other case.
+constexpr int U2P_DEFERRED = 1, U2P_DEFERRED_105 = 2;
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false>
 KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -641,9 +650,9 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
     uu[0] = P.rhofloor * gdetu;
     ret = -2;
   }
-  u2pret = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 0);
+  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD>(P, uu, pp, g, 0);
   if (u2pret < 0) {
-    if (DEFER) return U2P_DEFERRED;
+    if (DEFER) return u2pret == -105 ? U2P_DEFERRED_105 : U2P_DEFERRED;
     ret = -1;
     u2pret = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 1);
     if (u2pret < 0) ret = -2;
@@ -776,18 +785,20 @@ KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 // calc_primitives core (u2p.c:13-133) for an active, non-axis cell: returns
 // the u2p code; sets flags and updates uu to follow the floored primitives.
 // With DEFER a cell that needs the entropy inversion or the b^2/rho floor
-// returns CP_DEFERRED before writing uu (pp may be changed) and is redone by
-// the caller with DEFER false.
-constexpr int CP_DEFERRED = 100;
-template <int MAG, int RAD, bool DEFER = false>
+// returns CP_DEFERRED_HOT (its hot inversion failed and left pp as it was)
+// or CP_DEFERRED before writing uu (pp may be changed) and is redone by the
+// caller with DEFER false, with SKIP_HOT for CP_DEFERRED_HOT.
+constexpr int CP_DEFERRED = 100, CP_DEFERRED_HOT = 101;
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false>
 KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
                                int &fl_entr2, int &fl_hd, int &fl_rad)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};
   fl_entr = 0;
   fl_entr2 = 0;
-  int r = u2p<MAG, RAD, DEFER>(P, uu, pp, g, corrected, fixups);
-  if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED;
+  int r = u2p<MAG, RAD, DEFER, SKIP_HOT>(P, uu, pp, g, corrected, fixups);
+  if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED_HOT;
+  if (DEFER && r == U2P_DEFERRED_105) return CP_DEFERRED;
   if (corrected[0] == 1) fl_entr = 1;
   if (corrected[2] == 1) fl_entr2 = 1;
   if (floors_mhd<MAG, RAD, DEFER>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;

@@ -687,9 +687,11 @@ struct OpU2P {
   signed char *fl;
   unsigned *list, *cnt;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
-  KD_INL Cnt3 operator()(long i) const { return cell<true>(i, list, cnt); }
-  // the full calc_primitives (DEFER false) or the deferring one on cell i
-  template <bool DEFER>
+  unsigned char *why;  // per list entry: 1 when the redo may skip the failed hot inversion
+  KD_INL Cnt3 operator()(long i) const { return cell<true, false>(i, list, cnt); }
+  // the deferring calc_primitives, or the full one (optionally from the
+  // entropy inversion on) on cell i
+  template <bool DEFER, bool SKIP_HOT>
   KD_INL Cnt3 cell(long i, unsigned *lst, unsigned *len) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
@@ -703,10 +705,14 @@ struct OpU2P {
     Geom g;
     geom_cell(G, ix, iy, g);
     int fe, fe2, fh, fr;
-    const int r = calc_primitives<MAG, RAD, DEFER>(P, uu, pp, g, fe, fe2, fh, fr);
+    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT>(P, uu, pp, g, fe, fe2, fh, fr);
     if (DEFER) {
-      append_index(r == CP_DEFERRED, (unsigned)i, lst, len);
-      if (r == CP_DEFERRED) return Cnt3{{0, 0, 0}};
+      const bool defer = r == CP_DEFERRED || r == CP_DEFERRED_HOT;
+      const long slot = append_index(defer, (unsigned)i, lst, len);
+      if (defer) {
+        why[slot] = r == CP_DEFERRED_HOT ? 1 : 0;
+        return Cnt3{{0, 0, 0}};
+      }
     }
     store_p<MAG, RAD>(p, G.ncell, c, pp);
     store_p<MAG, RAD>(u, G.ncell, c, uu);
@@ -732,7 +738,8 @@ struct OpU2PFloor {
     Cnt3 sum{{0, 0, 0}};
     const long m = (long)*op.cnt;
     for (long j = t; j < m; j += FLOOR_THREADS) {
-      const Cnt3 c = op.template cell<false>((long)op.list[j], nullptr, nullptr);
+      const Cnt3 c = op.why[j] ? op.template cell<false, true>((long)op.list[j], nullptr, nullptr)
+                                : op.template cell<false, false>((long)op.list[j], nullptr, nullptr);
 #pragma unroll
       for (int k = 0; k < 3; k++) sum.v[k] += c.v[k];
     }
@@ -980,6 +987,7 @@ struct GridBufs {
   unsigned *flist[3];  // faces of each sweep left to the b^2/rho floor pass, face_states_cap(G)
   unsigned *fcnt;      // their counts [3], then the count of ulist
   unsigned *ulist;     // cells left to the full calc_primitives pass, nx ny nz
+  unsigned char *uwhy;  // per ulist entry: the redo may skip the failed hot inversion
   double *pn, *un;
   int *mark;
   double *ut0, *ut1, *uforget, *dut1, *dut2, *drt1, *drt2;
@@ -1169,7 +1177,7 @@ void u2p(Ex &ex, const GridBufs &s)
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
     ex.each(OpZeroU32{s.fcnt + 3, 1});
-    const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3};
+    const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy};
     ex.count(op, s.counters + 3);
     ex.count(OpU2PFloor<T::MAG, T::RAD>{op}, s.counters + 3);
   });

@@ -458,6 +458,7 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
     }
   if (!rc) rc = g->alloc(&s.fcnt, 4);
   if (!rc) rc = g->alloc(&s.ulist, (long)G.nx * G.ny * G.nz);
+  if (!rc) rc = g->alloc(&s.uwhy, (long)G.nx * G.ny * G.nz);
   if (!rc) rc = g->make_side();
   if (!rc) rc = g->alloc(&s.emf, 3L * (G.nx + 1) * (G.ny + 1) * (G.nz + 1));
   if (!rc) rc = g->alloc(&s.pn, nv * nc);

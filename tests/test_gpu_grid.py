@@ -178,7 +178,7 @@ def test_grid_step_parity(oracle, kind, nsteps):
         dg.dt = dt
 
 
-@pytest.mark.parametrize("kind", ["tube", "torus_mhd_polar"])
+@pytest.mark.parametrize("kind", ["tube", "torus_mhd_polar", "torus_mhd_bfloor"])
 def test_grid_run_matches_steps(kind):
     """Steps with the time kept on the device (koral_hip_run_rk2imex) give
     the same state, time, step and counters as steps driven from the host."""
@@ -202,7 +202,7 @@ def test_grid_run_matches_steps(kind):
     assert np.array_equal(tot, np.asarray(b.counters, np.int64))
 
 
-@pytest.mark.parametrize("kind", ["tube", "torus_mhd_polar"])
+@pytest.mark.parametrize("kind", ["tube", "torus_mhd_polar", "torus_mhd_bfloor"])
 def test_grid_graph_replay(kind):
     """The step captured as a HIP graph and replayed: three runs of two
     steps each, with eager work and synchronisations between the replays,
