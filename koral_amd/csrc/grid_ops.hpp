@@ -1105,7 +1105,12 @@ void explicit_op(Ex &ex, const GridBufs &s)
   const int tt = P.mag ? 1 : 0;
   const int n[3] = {G.nx, G.ny, G.nz};
   ex.each(OpZeroU32{s.fcnt, 3});
-  for (int d = 0; d < 3; d++) {
+  // x2 first: its floor pass (faces next to the magnetised torus surface,
+  // the longest) then overlaps the x1 and x3 sweeps.  The sweeps write
+  // separate flux arrays, so their order does not change any result.
+  const int order[3] = {1, 0, 2};
+  for (int od = 0; od < 3; od++) {
+    const int d = order[od];
     if (n[d] == 1) continue;
     int lo[3], cnt[3];
     for (int k = 0; k < 3; k++) {
