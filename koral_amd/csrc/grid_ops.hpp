@@ -1094,13 +1094,15 @@ void tstepden(Ex &ex, const GridBufs &s)
 }
 
 // op_explicit: ghosts, wave speeds (tsd accumulates the max), face fluxes,
-// flux-CT, update
+// flux-CT, update.  ghosts_current: the ghost cells already hold set_bc of
+// the current p and u (the ghost fill is a function of the interior alone,
+// so a second fill would write the same values) and are not refilled.
 template <class Ex>
-void explicit_op(Ex &ex, const GridBufs &s)
+void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false)
 {
   const DGrid &G = s.G;
   const koral_params &P = s.P;
-  set_bc(ex, s);
+  if (!ghosts_current) set_bc(ex, s);
   tstepden(ex, s);
   const int tt = P.mag ? 1 : 0;
   const int n[3] = {G.nx, G.ny, G.nz};
@@ -1191,6 +1193,9 @@ void u2p(Ex &ex, const GridBufs &s)
   set_bc(ex, s);
 }
 
+// whether do_correct changes anything (ko.c calls it after every calc_u2p)
+inline bool polar_correction_active(const GridBufs &s) { return s.P.polar_nc > 0 && s.G.ny >= 2; }
+
 // do_correct (finite.c:605-636) with CORRECT_POLARAXIS: correct_polaraxis
 // for spherical-like coordinates (finite.c:4924-5060).  One item per
 // (ix, iz, face, ic): the cell ic away from the polar face takes the gas and
@@ -1252,7 +1257,7 @@ struct OpPolar {
 template <class Ex>
 void do_correct(Ex &ex, const GridBufs &s)
 {
-  if (s.P.polar_nc <= 0 || s.G.ny < 2) return;
+  if (!polar_correction_active(s)) return;
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
     ex.each(OpPolar<T::MAG, T::RAD>{s.P, s.G, s.p, s.u});
@@ -1307,7 +1312,8 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   seq({CombStmt{s.drt1, 2, {A(idtg, s.u), A(midtg, s.ut0), none}}, CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
   u2p(ex, s);
   do_correct(ex, s);
-  explicit_op(ex, s);
+  // u2p ended with set_bc; without a polar correction p and u are unchanged
+  explicit_op(ex, s, !polar_correction_active(s));
   // dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
   seq({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
        CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
@@ -1320,7 +1326,8 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
        CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
   u2p(ex, s);
   do_correct(ex, s);
-  explicit_op(ex, s);
+  // u2p ended with set_bc; without a polar correction p and u are unchanged
+  explicit_op(ex, s, !polar_correction_active(s));
   // dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
   seq({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
        CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
