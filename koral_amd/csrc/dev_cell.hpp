@@ -307,7 +307,23 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 // Wstart > 0 replaces the W start computed from pp (the learned W guess,
 // which has no reference counterpart); iters, when given, receives the
 // Newton iterations made.
-template <int MAG, int RAD>
+//
+// WAVE: all 64 lanes of a wave run the inversion of one point (uniform
+// inputs).  The two trial loops of the iteration -- W *= 10 until the
+// residual is admissible (at most 50 times), and the step halving until the
+// trial W is admissible (at most 100 halvings) -- are searches for the
+// first admissible trial; the lanes evaluate one trial each (trial k is
+// formed with the same k sequential multiplications or halvings) and take
+// the first admissible one by ballot, which is where the sequential loop
+// stops, so the result is the same bit for bit.  A point whose iteration
+// halves its step many times then costs a few evaluations of the residual
+// instead of up to a hundred in sequence.
+#ifdef KD_HOST_CHECK
+#define KD_WAVE_SOLVER 0
+#else
+#define KD_WAVE_SOLVER 1
+#endif
+template <int MAG, int RAD, bool WAVE = false>
 KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype,
                         double Wstart = 0., int *iters = nullptr)
 {
@@ -358,17 +374,35 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   double f0, dfdW, err;
   int i_increase = 0;
   if (iters) *iters = 0;
-  for (;;) {
-    f0 = dfdW = 0.;
-    f_u2p(P, etype, W - D, c, f0, dfdW, err);
-    if ((w_bad(W, Bsq, QdotBsq, Qtsq) || !isfinite(f0) || !isfinite(dfdW)) && i_increase < 50) {
-      W *= 10.;
-      i_increase++;
-      continue;
+#if KD_WAVE_SOLVER
+  if constexpr (WAVE) {
+    const int lane = threadIdx.x & 63;
+    double Wk = W;
+    for (int k = 0; k < lane; k++) Wk *= 10.;
+    bool ok = false;
+    if (lane < 50) {
+      double f = 0., df = 0., e;
+      f_u2p(P, etype, Wk - D, c, f, df, e);
+      ok = !(w_bad(Wk, Bsq, QdotBsq, Qtsq) || !isfinite(f) || !isfinite(df));
     }
-    break;
+    const unsigned long long m = __ballot(ok);
+    if (!m) return -150;
+    W = __shfl(Wk, __ffsll(m) - 1);
+  } else
+#endif
+  {
+    for (;;) {
+      f0 = dfdW = 0.;
+      f_u2p(P, etype, W - D, c, f0, dfdW, err);
+      if ((w_bad(W, Bsq, QdotBsq, Qtsq) || !isfinite(f0) || !isfinite(dfdW)) && i_increase < 50) {
+        W *= 10.;
+        i_increase++;
+        continue;
+      }
+      break;
+    }
+    if (i_increase >= 50) return -150;
   }
-  if (i_increase >= 50) return -150;
   int iter = 0;
   double Wprev;
   do {
@@ -382,20 +416,47 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
       continue;
     }
     double Wnew = W - f0 / dfdW;
-    int idump = 0;
-    double dumpfac = 1.;
-    for (;;) {
-      double f0t = 0., dft = 0., errt;
-      f_u2p(P, etype, Wnew - D, c, f0t, dft, errt);
-      if ((w_bad(Wnew, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft)) && idump < 100) {
-        idump++;
-        dumpfac /= 2.;
-        Wnew = W - dumpfac * f0 / dfdW;
-        continue;
+#if KD_WAVE_SOLVER
+    if constexpr (WAVE) {
+      // trials k = 0..99 in two rounds of 64; trial k halves the step k times
+      const int lane = threadIdx.x & 63;
+      int found = -1;
+      for (int round = 0; round < 2 && found < 0; round++) {
+        const int k = round * 64 + lane;
+        double dumpfac = 1.;
+        for (int q = 0; q < k; q++) dumpfac /= 2.;
+        const double Wt = k == 0 ? Wnew : W - dumpfac * f0 / dfdW;
+        bool ok = false;
+        if (k < 100) {
+          double f0t = 0., dft = 0., errt;
+          f_u2p(P, etype, Wt - D, c, f0t, dft, errt);
+          ok = !(w_bad(Wt, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft));
+        }
+        const unsigned long long m = __ballot(ok);
+        if (m) {
+          found = round * 64 + __ffsll(m) - 1;
+          Wnew = __shfl(Wt, __ffsll(m) - 1);
+        }
       }
-      break;
+      if (found < 0) return -101;
+    } else
+#endif
+    {
+      int idump = 0;
+      double dumpfac = 1.;
+      for (;;) {
+        double f0t = 0., dft = 0., errt;
+        f_u2p(P, etype, Wnew - D, c, f0t, dft, errt);
+        if ((w_bad(Wnew, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft)) && idump < 100) {
+          idump++;
+          dumpfac /= 2.;
+          Wnew = W - dumpfac * f0 / dfdW;
+          continue;
+        }
+        break;
+      }
+      if (idump >= 100) return -101;
     }
-    if (idump >= 100) return -101;
     W = Wnew;
     if (fabs(W) > BIG) return -103;
     if (fabs((W - Wprev) / Wprev) < P.u2pconv && err < 1.e-1) break;
@@ -630,7 +691,7 @@ KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 // entropy inversion (SKIP_HOT) and reach the same state; U2P_DEFERRED_105
 // marks the other case.
 constexpr int U2P_DEFERRED = 1, U2P_DEFERRED_105 = 2;
-template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false>
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
 KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -650,11 +711,11 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
     uu[0] = P.rhofloor * gdetu;
     ret = -2;
   }
-  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD>(P, uu, pp, g, 0);
+  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0);
   if (u2pret < 0) {
     if (DEFER) return u2pret == -105 ? U2P_DEFERRED_105 : U2P_DEFERRED;
     ret = -1;
-    u2pret = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 1);
+    u2pret = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 1);
     if (u2pret < 0) ret = -2;
   }
   if (u2pret < 0) {
@@ -679,7 +740,7 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
 // two inlined Newton inversions of that rare branch out of a kernel's
 // common path is what DEFER is for.
 constexpr int FLOORS_DEFERRED = 1;
-template <int MAG, int RAD, bool DEFER = false>
+template <int MAG, int RAD, bool DEFER = false, bool WAVE = false>
 KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -725,8 +786,8 @@ KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
       p2u_mhd<MAG, RAD>(P, dpp, duu, g);
 #pragma unroll
       for (int i = 0; i < 9; i++) uu[i] += duu[i];
-      int rt = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 0);
-      if (rt < 0) rt = u2p_solver_W<MAG, RAD>(P, uu, pp, g, 1);
+      int rt = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0);
+      if (rt < 0) rt = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 1);
       ret = -1;
     }
   }
@@ -789,19 +850,19 @@ KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 // or CP_DEFERRED before writing uu (pp may be changed) and is redone by the
 // caller with DEFER false, with SKIP_HOT for CP_DEFERRED_HOT.
 constexpr int CP_DEFERRED = 100, CP_DEFERRED_HOT = 101;
-template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false>
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
 KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
                                int &fl_entr2, int &fl_hd, int &fl_rad)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};
   fl_entr = 0;
   fl_entr2 = 0;
-  int r = u2p<MAG, RAD, DEFER, SKIP_HOT>(P, uu, pp, g, corrected, fixups);
+  int r = u2p<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, corrected, fixups);
   if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED_HOT;
   if (DEFER && r == U2P_DEFERRED_105) return CP_DEFERRED;
   if (corrected[0] == 1) fl_entr = 1;
   if (corrected[2] == 1) fl_entr2 = 1;
-  if (floors_mhd<MAG, RAD, DEFER>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;
+  if (floors_mhd<MAG, RAD, DEFER, WAVE>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;
   if (RAD) floors_rad<MAG, RAD>(P, pp, g);
   p2u<MAG, RAD>(P, pp, uu, g);
   fl_hd = (fixups[0] > 0) ? 1 : 0;

@@ -688,11 +688,12 @@ struct OpU2P {
   unsigned *list, *cnt;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   unsigned char *why;  // per list entry: 1 when the redo may skip the failed hot inversion
-  KD_INL Cnt3 operator()(long i) const { return cell<true, false>(i, list, cnt); }
+  KD_INL Cnt3 operator()(long i) const { return cell<true, false, false>(i, list, cnt, true); }
   // the deferring calc_primitives, or the full one (optionally from the
-  // entropy inversion on) on cell i
-  template <bool DEFER, bool SKIP_HOT>
-  KD_INL Cnt3 cell(long i, unsigned *lst, unsigned *len) const
+  // entropy inversion on) on cell i; WAVE: the calling wave's lanes all run
+  // cell i together and only the lane with store writes it
+  template <bool DEFER, bool SKIP_HOT, bool WAVE>
+  KD_INL Cnt3 cell(long i, unsigned *lst, unsigned *len, bool store) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     Box b{0, 0, 0, G.nx, G.ny, G.nz};
@@ -705,7 +706,7 @@ struct OpU2P {
     Geom g;
     geom_cell(G, ix, iy, g);
     int fe, fe2, fh, fr;
-    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT>(P, uu, pp, g, fe, fe2, fh, fr);
+    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, fe, fe2, fh, fr);
     if (DEFER) {
       const bool defer = r == CP_DEFERRED || r == CP_DEFERRED_HOT;
       const long slot = append_index(defer, (unsigned)i, lst, len);
@@ -714,6 +715,7 @@ struct OpU2P {
         return Cnt3{{0, 0, 0}};
       }
     }
+    if (!store) return Cnt3{{0, 0, 0}};
     store_p<MAG, RAD>(p, G.ncell, c, pp);
     store_p<MAG, RAD>(u, G.ncell, c, uu);
     const long n = G.ncell;
@@ -725,21 +727,25 @@ struct OpU2P {
   }
 };
 
-// the cells OpU2P left, with the full calc_primitives; FLOOR_THREADS
-// threads stride over the list, whose length is known only on the device
+// the cells OpU2P left, with the full calc_primitives, one wave per cell
+// (the W inversions evaluate their trials one per lane); U2P_FLOOR_WAVES
+// waves stride over the list, whose length is known only on the device
+constexpr long U2P_FLOOR_WAVES = 1024;
 template <int MAG, int RAD>
 struct OpU2PFloor {
   static constexpr int kind = K_U2P_FLOOR;
   static constexpr int waves_per_simd = 2;
   OpU2P<MAG, RAD> op;
-  KD_HD long size() const { return FLOOR_THREADS; }
+  KD_HD long size() const { return 64 * U2P_FLOOR_WAVES; }
   KD_INL Cnt3 operator()(long t) const
   {
     Cnt3 sum{{0, 0, 0}};
     const long m = (long)*op.cnt;
-    for (long j = t; j < m; j += FLOOR_THREADS) {
-      const Cnt3 c = op.why[j] ? op.template cell<false, true>((long)op.list[j], nullptr, nullptr)
-                                : op.template cell<false, false>((long)op.list[j], nullptr, nullptr);
+    const bool lead = (t & 63) == 0;
+    for (long j = t >> 6; j < m; j += U2P_FLOOR_WAVES) {
+      const long i = (long)op.list[j];
+      const Cnt3 c = op.why[j] ? op.template cell<false, true, true>(i, nullptr, nullptr, lead)
+                                : op.template cell<false, false, true>(i, nullptr, nullptr, lead);
 #pragma unroll
       for (int k = 0; k < 3; k++) sum.v[k] += c.v[k];
     }
