@@ -416,7 +416,7 @@ struct OpRecon {
 // of the two cells (f_calc_fluxes_at_faces finite.c:1297-1415)
 template <int MAG, int RAD, int D>
 KD_INL void face_flux(const koral_params &P, const DGrid &G, const double *__restrict__ ws, double *flb, int ix,
-                      int iy, int iz, const double *pL, const double *pR, const Geom &g)
+                      int iy, int iz, const double *pL, const double *pR, const Geom &g, bool store = true)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   constexpr int NVMHD = 6 + 3 * MAG;
@@ -446,7 +446,7 @@ KD_INL void face_flux(const koral_params &P, const DGrid &G, const double *__res
       else
         F = (-al * fR[iv] + ar * fL[iv] + al * ar * (uR[iv] - uL[iv])) / (ar - al);
     }
-    flb[iv * nf + f] = F;
+    if (store) flb[iv * nf + f] = F;
   }
 }
 
@@ -497,9 +497,10 @@ struct OpFlux {
 };
 
 // The faces OpFlux left: the same face states, the full face floors, the
-// same fluxes.  FLOOR_THREADS threads stride over the list, whose length is
-// known only on the device.
-constexpr long FLOOR_THREADS = 16384;
+// same fluxes, one wave per face (the floors' W inversions evaluate their
+// trials one per lane; lane 0 stores).  FLOOR_WAVES waves stride over the
+// list, whose length is known only on the device.
+constexpr long FLOOR_WAVES = 1024;
 template <int MAG, int RAD, int D>
 struct OpFluxFloor {
   static constexpr int kind = K_FLUX_FLOOR;
@@ -510,12 +511,13 @@ struct OpFluxFloor {
   double *flb;
   const unsigned *list, *cnt;
   Box b;
-  KD_HD long size() const { return FLOOR_THREADS; }
+  KD_HD long size() const { return 64 * FLOOR_WAVES; }
   KD_INL void operator()(long t) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     const long m = (long)*cnt;
-    for (long j = t; j < m; j += FLOOR_THREADS) {
+    const bool lead = (t & 63) == 0;
+    for (long j = t >> 6; j < m; j += FLOOR_WAVES) {
       const long i = list[j];
       int ix, iy, iz;
       b.at(i, ix, iy, iz);
@@ -523,9 +525,9 @@ struct OpFluxFloor {
       load_face_states<NV>(fs, b.size(), i, pL, pR);
       Geom g;
       geom_face(G, D, ix, iy, g);
-      floors_mhd<MAG, RAD>(P, pL, g);
-      floors_mhd<MAG, RAD>(P, pR, g);
-      face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g);
+      floors_mhd<MAG, RAD, false, true>(P, pL, g);
+      floors_mhd<MAG, RAD, false, true>(P, pR, g);
+      face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g, lead);
     }
   }
 };
