@@ -313,7 +313,8 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 // residual is admissible (at most 50 times), and the step halving until the
 // trial W is admissible (at most 100 halvings) -- are searches for the
 // first admissible trial; the lanes evaluate one trial each (trial k is
-// formed with the same k sequential multiplications or halvings) and take
+// formed with the same k sequential multiplications, or as 2^-k, which k
+// halvings of 1 give exactly) and take
 // the first admissible one by ballot, which is where the sequential loop
 // stops, so the result is the same bit for bit.  A point whose iteration
 // halves its step many times then costs a few evaluations of the residual
@@ -423,8 +424,8 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
       int found = -1;
       for (int round = 0; round < 2 && found < 0; round++) {
         const int k = round * 64 + lane;
-        double dumpfac = 1.;
-        for (int q = 0; q < k; q++) dumpfac /= 2.;
+        // k halvings of 1 are exact: 2^-k
+        const double dumpfac = ldexp(1., -k);
         const double Wt = k == 0 ? Wnew : W - dumpfac * f0 / dfdW;
         bool ok = false;
         if (k < 100) {
