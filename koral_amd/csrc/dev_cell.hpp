@@ -318,7 +318,8 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 // the first admissible one by ballot, which is where the sequential loop
 // stops, so the result is the same bit for bit.  A point whose iteration
 // halves its step many times then costs a few evaluations of the residual
-// instead of up to a hundred in sequence.
+// instead of up to a hundred in sequence; the residual at the accepted trial
+// also serves as the next iteration's residual.
 #ifdef KD_HOST_CHECK
 #define KD_WAVE_SOLVER 0
 #else
@@ -406,11 +407,22 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   }
   int iter = 0;
   double Wprev;
+  // WAVE: the residual at the accepted trial, evaluated by the winning lane,
+  // is the next iteration's residual at W (same inputs, same values)
+  bool have = false;
+  double fc = 0., dfc = 0., ec = 0.;
   do {
     Wprev = W;
     iter++;
     if (iters) *iters = iter;
-    f_u2p(P, etype, W - D, c, f0, dfdW, err);
+    if (WAVE && have) {
+      f0 = fc;
+      dfdW = dfc;
+      err = ec;
+    } else {
+      f_u2p(P, etype, W - D, c, f0, dfdW, err);
+    }
+    have = false;
     if (err < P.u2pconv) break;
     if (dfdW == 0.) {
       W *= 1.1;
@@ -428,15 +440,20 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
         const double dumpfac = ldexp(1., -k);
         const double Wt = k == 0 ? Wnew : W - dumpfac * f0 / dfdW;
         bool ok = false;
+        double f0t = 0., dft = 0., errt = 0.;
         if (k < 100) {
-          double f0t = 0., dft = 0., errt;
           f_u2p(P, etype, Wt - D, c, f0t, dft, errt);
           ok = !(w_bad(Wt, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft));
         }
         const unsigned long long m = __ballot(ok);
         if (m) {
-          found = round * 64 + __ffsll(m) - 1;
-          Wnew = __shfl(Wt, __ffsll(m) - 1);
+          const int win = __ffsll(m) - 1;
+          found = round * 64 + win;
+          Wnew = __shfl(Wt, win);
+          fc = __shfl(f0t, win);
+          dfc = __shfl(dft, win);
+          ec = __shfl(errt, win);
+          have = true;
         }
       }
       if (found < 0) return -101;

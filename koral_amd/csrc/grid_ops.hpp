@@ -732,7 +732,7 @@ struct OpU2P {
 // the cells OpU2P left, with the full calc_primitives, one wave per cell
 // (the W inversions evaluate their trials one per lane); U2P_FLOOR_WAVES
 // waves stride over the list, whose length is known only on the device
-constexpr long U2P_FLOOR_WAVES = 1024;
+constexpr long U2P_FLOOR_WAVES = 4096;
 template <int MAG, int RAD>
 struct OpU2PFloor {
   static constexpr int kind = K_U2P_FLOOR;
