@@ -408,7 +408,9 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   int iter = 0;
   double Wprev;
   // WAVE: the residual at the accepted trial, evaluated by the winning lane,
-  // is the next iteration's residual at W (same inputs, same values)
+  // is the next iteration's residual at W (same inputs, same values).  The
+  // sequential path evaluates it again: carrying it there lengthened
+  // register lifetimes in the register-bound implicit kernels (3 % slower).
   bool have = false;
   double fc = 0., dfc = 0., ec = 0.;
   do {
