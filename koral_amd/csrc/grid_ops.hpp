@@ -411,6 +411,65 @@ struct OpRecon {
   }
 };
 
+// OpRecon for MP5 across x2 or x3 (D = 1, 2), a thread marching over a run
+// of RUN faces along D: the six-cell window of each variable slides by one
+// cell per face, so each cell is loaded once per run instead of six times
+// (the stencil cells of these sweeps lie rows or planes apart and were
+// re-read from HBM).  The same values reach mp5 in the same order.
+template <int MAG, int RAD, int D>
+struct OpReconRun {
+  static constexpr int kind = K_RECON;
+  static constexpr int RUN = 4;
+  koral_params P;
+  DGrid G;
+  const double *p;
+  double *fs;
+  Box b;
+  KD_HD int len() const { return D == 1 ? b.ny : b.nz; }
+  KD_HD int runs() const { return (len() + RUN - 1) / RUN; }
+  // items: x fastest, then the other transverse index, then the run
+  KD_HD long size() const { return (long)b.nx * (D == 1 ? b.nz : b.ny) * runs(); }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    const int ix = b.x0 + (int)(t % b.nx);
+    const long r1 = t / b.nx;
+    const int nt = D == 1 ? b.nz : b.ny;
+    const int it = (int)(r1 % nt);
+    const int run = (int)(r1 / nt);
+    const int j0 = run * RUN;
+    const int iy0 = D == 1 ? b.y0 + j0 : b.y0 + it;
+    const int iz0 = D == 1 ? b.z0 + it : b.z0 + j0;
+    const long n = G.ncell, nb = b.size();
+    const long st = dir_stride<D>(G);
+    const long c0 = cidx(G, ix, iy0, iz0);
+    const int m = len() - j0 < RUN ? len() - j0 : RUN;
+    // box index of the run's first face and its step along D
+    const long i0 = (ix - b.x0) + (long)b.nx * ((iy0 - b.y0) + (long)b.ny * (iz0 - b.z0));
+    const long di = D == 1 ? (long)b.nx : (long)b.nx * b.ny;
+#pragma unroll 1
+    for (int iv = 0; iv < NV; iv++) {
+      const double *q = p + iv * n + c0;
+      double v[6];
+#pragma unroll
+      for (int o = 0; o < 6; o++) v[o] = q[(o - 3) * st];
+#pragma unroll
+      for (int j = 0; j < RUN; j++) {
+        if (j < m) {
+          const long i = i0 + j * di;
+          fs[iv * nb + i] = mp5(v[0], v[1], v[2], v[3], v[4]);
+          fs[(NV + iv) * nb + i] = mp5(v[5], v[4], v[3], v[2], v[1]);
+          if (j + 1 < m) {
+#pragma unroll
+            for (int o = 0; o < 5; o++) v[o] = v[o + 1];
+            v[5] = q[(j + 3) * st];
+          }
+        }
+      }
+    }
+  }
+};
+
 // Fluxes at the face (ix, iy, iz) of direction D from its floored face
 // states: physical fluxes, conserved states, HLL or LF with the wave speeds
 // of the two cells (f_calc_fluxes_at_faces finite.c:1297-1415)
@@ -1134,7 +1193,9 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false)
       constexpr int M = T::MAG, R = T::RAD;
       auto sweep = [&](auto dd) {
         constexpr int D = decltype(dd)::value;
-        if (P.int_order == 4)
+        if (P.int_order == 4 && D > 0)
+          ex.each(OpReconRun<M, R, (D > 0 ? D : 1)>{P, G, s.p, s.fs[D], fb});
+        else if (P.int_order == 4)
           ex.each(OpRecon<M, R, 4, D>{P, G, s.p, s.fs[D], fb});
         else if (P.int_order == 2)
           ex.each(OpRecon<M, R, 2, D>{P, G, s.p, s.fs[D], fb});
