@@ -34,7 +34,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # recon averaged over the three sweep directions): FETCH_SIZE and WRITE_SIZE
 # per dispatch scaled by the same counters on a plain copy of known size
 # (profiles/hbm_traffic_grmhd_torus_128x128x64.md).
-GRID_TRAFFIC_128x128x64 = {"u2p": 315e6, "flux": 445e6, "recon": 425e6}
+GRID_TRAFFIC_128x128x64 = {"u2p": 315e6, "flux": 445e6, "recon": 323e6}
 
 # Algorithmic FP64 operations per implicit solve of the seeded batch workload
 # (all six fallback attempts included), counted by the operation-counting
