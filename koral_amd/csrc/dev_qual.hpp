@@ -16,3 +16,12 @@
 #define KD_FUN __device__
 #define KD_HD __host__ __device__ inline
 #endif
+
+// Compiler-only ordering point: no memory access is moved across it, which
+// bounds how many loads the scheduler hoists (and keeps live) at once.  It
+// emits no instruction.
+#ifdef KD_HOST_CHECK
+inline void compiler_fence() { asm volatile("" ::: "memory"); }
+#else
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+#endif

@@ -695,6 +695,7 @@ struct OpCtFaces {
 template <int MAG, int RAD>
 struct OpUpdate {
   static constexpr int kind = K_UPDATE;
+  static constexpr int waves_per_simd = RAD ? 3 : 4;
   koral_params P;
   DGrid G;
   const double *p;
@@ -710,11 +711,11 @@ struct OpUpdate {
     int ix, iy, iz;
     b.at(i, ix, iy, iz);
     long c = cidx(G, ix, iy, iz);
-    double pp[NV], ms[NV];
-    load_p<MAG, RAD>(p, G.ncell, c, pp);
-    Geom g;
-    geom_cell(G, ix, iy, g);
-    metric_source<MAG, RAD>(P, G, pp, ix, iy, g, ms);
+    // flux divergence first, metric source second: the compiler fence keeps
+    // the 64 Christoffel and 40 metric loads from being hoisted beside the
+    // 6 NV flux loads (253 VGPRs, 2 waves per SIMD otherwise); the
+    // arithmetic of every term is unchanged
+    double du[NV];
     double dx = size_x(G, ix, 0), dy = size_x(G, iy, 1), dz = size_x(G, iz, 2);
     long fxl = fxidx(G, ix, iy, iz), fxr = fxidx(G, ix + 1, iy, iz);
     long fyl = 0, fyr = 0, fzl = 0, fzr = 0;
@@ -722,11 +723,18 @@ struct OpUpdate {
     if (G.nz > 1) { fzl = fzidx(G, ix, iy, iz); fzr = fzidx(G, ix, iy, iz + 1); }
 #pragma unroll
     for (int iv = 0; iv < NV; iv++) {
-      double du = -(F.x[iv * F.nfx + fxr] - F.x[iv * F.nfx + fxl]) * dt / dx;
-      if (G.ny > 1) du = du - (F.y[iv * F.nfy + fyr] - F.y[iv * F.nfy + fyl]) * dt / dy;
-      if (G.nz > 1) du = du - (F.z[iv * F.nfz + fzr] - F.z[iv * F.nfz + fzl]) * dt / dz;
-      u[iv * G.ncell + c] = u[iv * G.ncell + c] + du + ms[iv] * dt;
+      du[iv] = -(F.x[iv * F.nfx + fxr] - F.x[iv * F.nfx + fxl]) * dt / dx;
+      if (G.ny > 1) du[iv] = du[iv] - (F.y[iv * F.nfy + fyr] - F.y[iv * F.nfy + fyl]) * dt / dy;
+      if (G.nz > 1) du[iv] = du[iv] - (F.z[iv * F.nfz + fzr] - F.z[iv * F.nfz + fzl]) * dt / dz;
     }
+    compiler_fence();
+    double pp[NV], ms[NV];
+    load_p<MAG, RAD>(p, G.ncell, c, pp);
+    Geom g;
+    geom_cell(G, ix, iy, g);
+    metric_source<MAG, RAD>(P, G, pp, ix, iy, g, ms);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) u[iv * G.ncell + c] = u[iv * G.ncell + c] + du[iv] + ms[iv] * dt;
   }
 };
 
