@@ -145,12 +145,13 @@ struct OpBcFaces {
 };
 
 // Ghosts along the four a-b edges of the domain, at every position k along
-// the third dimension c: finite.c:3336-3600 recipe, in two launches that keep
-// the reference's data dependences.  Stage A: the copies of the far face
-// ghosts (NG-1 deep on each side of the edge) and the average next to the
-// corner, (ea, eb) = <(ea, ib), (ia, eb)>, all reading face ghosts only.
-// Stage B: the diagonal average (ea2, eb2) = <(ea2, eb), (ea, eb2)> of two
-// stage-A copies.  Each written cell gets u = p2u(p).
+// the third dimension c: finite.c:3336-3600 recipe, in one launch.  The copies of the far face ghosts (NG-1 deep on each side of
+// the edge) and the average next to the corner, (ea, eb) = <(ea, ib),
+// (ia, eb)>, read face ghosts only.  The reference then averages two of those
+// copies on the diagonal, (ea2, eb2) = <(ea2, eb), (ea, eb2)>; the copies are
+// exact, so the same item reads their sources (ea2, ib) and (ia, eb2)
+// instead, with the same operand order, and the launch has no internal
+// dependence.  Each written cell gets u = p2u(p); NG >= 2.
 template <int MAG, int RAD>
 struct EdgeGeo {
   DGrid G;
@@ -209,7 +210,7 @@ struct OpBcEdgeA {
   koral_params P;
   EdgeGeo<MAG, RAD> E;
   double *p, *u;
-  KD_HD int per() const { return 2 * (E.G.ng - 1) + 1; }  // items per (corner, k)
+  KD_HD int per() const { return 2 * (E.G.ng - 1) + 2; }  // items per (corner, k)
   KD_HD long size() const { return 4L * E.nk() * per(); }
   KD_INL void operator()(long t) const
   {
@@ -229,28 +230,12 @@ struct OpBcEdgeA {
       const int i = m - (NG - 1);
       const int gb = bhi ? nb + i + 1 : -NG + i;
       edge_set<MAG, RAD>(P, E, p, u, k, ea, gb, ia, gb, 0, 0, false);
-    } else {
+    } else if (m == 2 * (NG - 1)) {
       edge_set<MAG, RAD>(P, E, p, u, k, ea, eb, ea, ib, ia, eb, true);
+    } else {
+      const int ea2 = ahi ? na + 1 : -2, eb2 = bhi ? nb + 1 : -2;
+      edge_set<MAG, RAD>(P, E, p, u, k, ea2, eb2, ea2, ib, ia, eb2, true);
     }
-  }
-};
-
-template <int MAG, int RAD>
-struct OpBcEdgeB {
-  static constexpr int kind = K_BC_EDGE;
-  koral_params P;
-  EdgeGeo<MAG, RAD> E;
-  double *p, *u;
-  KD_HD long size() const { return 4L * E.nk(); }
-  KD_INL void operator()(long t) const
-  {
-    const int k = -E.ngc() + (int)(t % E.nk());
-    const int corner = (int)(t / E.nk());
-    const int ahi = corner >> 1, bhi = corner & 1;
-    const int na = dimn(E.G, E.a), nb = dimn(E.G, E.b);
-    const int ea = ahi ? na : -1, ea2 = ahi ? na + 1 : -2;
-    const int eb = bhi ? nb : -1, eb2 = bhi ? nb + 1 : -2;
-    edge_set<MAG, RAD>(P, E, p, u, k, ea2, eb2, ea2, eb, ea, eb2, true);
   }
 };
 
@@ -1146,7 +1131,6 @@ void set_bc(Ex &ex, const GridBufs &s)
     auto edges = [&](int a, int b) {
       EdgeGeo<M, R> E{G, a, b};
       ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u});
-      ex.each(OpBcEdgeB<M, R>{s.P, E, s.p, s.u});
     };
     if (G.ny > 1) {
       edges(0, 1);
