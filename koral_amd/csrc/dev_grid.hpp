@@ -52,7 +52,7 @@ KD_INL long midx(const DGrid &G, int ix, int iy)
 {
   return (long)(G.ny > 1 ? iy + G.ngy : 0) * G.sx + ix + G.ngx;
 }
-KD_INL int dimn(const DGrid &G, int d) { return d == 0 ? G.nx : (d == 1 ? G.ny : G.nz); }
+KD_HD int dimn(const DGrid &G, int d) { return d == 0 ? G.nx : (d == 1 ? G.ny : G.nz); }
 
 KD_INL double size_x(const DGrid &G, int ic, int d)
 {

@@ -99,23 +99,34 @@ struct OpBcFaces {
   koral_params P;
   DGrid G;
   double *p, *u;
-  Box b;
-  KD_HD long size() const { return b.size(); }
+  // the items are the face-ghost cells only: for each direction d that is
+  // not collapsed, its 2 NG_d ghost layers over the interior of the other
+  // two directions (the cells with exactly one index outside the domain)
+  KD_HD int ngd(int d) const { return d == 0 ? G.ngx : (d == 1 ? G.ngy : G.ngz); }
+  KD_HD long slab(int d) const
+  {
+    return dimn(G, d) > 1 ? 2L * ngd(d) * (d == 0 ? 1L : G.nx) * (d == 1 ? 1L : G.ny) * (d == 2 ? 1L : G.nz) : 0;
+  }
+  KD_HD long size() const { return slab(0) + slab(1) + slab(2); }
   KD_INL void operator()(long i) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
+    int d = 0;
+    if (i >= slab(0)) {
+      i -= slab(0);
+      d = 1;
+      if (i >= slab(1)) {
+        i -= slab(1);
+        d = 2;
+      }
+    }
+    const int n = dimn(G, d), ng = ngd(d);
+    const Box b{0, 0, 0, d == 0 ? 2 * ng : G.nx, d == 1 ? 2 * ng : G.ny, d == 2 ? 2 * ng : G.nz};
     int ii[3];
     b.at(i, ii[0], ii[1], ii[2]);
-    int d = -1, hi = 0, out = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      int n = dimn(G, k);
-      if (n > 1 && ii[k] < 0) { d = k; hi = 0; out++; }
-      if (n > 1 && ii[k] >= n) { d = k; hi = 1; out++; }
-    }
-    if (out != 1) return;
+    const int hi = ii[d] >= ng;
+    ii[d] = hi ? n + ii[d] - ng : ii[d] - ng;
     int src[3] = {ii[0], ii[1], ii[2]};
-    int n = dimn(G, d);
     int type = G.bc[2 * d + hi];
     bool flip = false;
     if (type == BC_PERIODIC) {
@@ -1127,7 +1138,7 @@ void set_bc(Ex &ex, const GridBufs &s)
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
     constexpr int M = T::MAG, R = T::RAD;
-    ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u, Box{-G.ngx, -G.ngy, -G.ngz, G.sx, G.sy, G.sz}});
+    ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u});
     auto edges = [&](int a, int b) {
       EdgeGeo<M, R> E{G, a, b};
       ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u});
