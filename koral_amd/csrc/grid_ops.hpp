@@ -221,15 +221,17 @@ struct OpBcEdgeA {
   koral_params P;
   EdgeGeo<MAG, RAD> E;
   double *p, *u;
+  bool inner_k;  // positions k along the edge: the interior ones only
   KD_HD int per() const { return 2 * (E.G.ng - 1) + 2; }  // items per (corner, k)
-  KD_HD long size() const { return 4L * E.nk() * per(); }
+  KD_HD int nkk() const { return inner_k ? E.nk() - 2 * E.ngc() : E.nk(); }
+  KD_HD long size() const { return 4L * nkk() * per(); }
   KD_INL void operator()(long t) const
   {
     const int NG = E.G.ng;
     const int m = (int)(t % per());
     const long r = t / per();
-    const int k = -E.ngc() + (int)(r % E.nk());
-    const int corner = (int)(r / E.nk());
+    const int k = (inner_k ? 0 : -E.ngc()) + (int)(r % nkk());
+    const int corner = (int)(r / nkk());
     const int ahi = corner >> 1, bhi = corner & 1;
     const int na = dimn(E.G, E.a), nb = dimn(E.G, E.b);
     const int ea = ahi ? na : -1, ia = ahi ? na - 1 : 0;
@@ -274,6 +276,25 @@ struct OpBcTotal {
 #pragma unroll
     for (int iv = 0; iv < NV; iv++) p[iv * G.ncell + d] = p[iv * G.ncell + s];
     p2u_at<MAG, RAD>(P, G, p, u, ix, iy, iz);
+  }
+};
+
+// the last edge pair (y-z, along x) and the corner blocks in one launch:
+// OpBcTotal reads interior cells only and overwrites every cell that the pair
+// writes at ghost positions along x, so those edge items are left out and the
+// launch has no internal dependence
+template <int MAG, int RAD>
+struct OpBcEdgeTotal {
+  static constexpr int kind = K_BC_EDGE;
+  OpBcEdgeA<MAG, RAD> e;  // E = (1, 2), inner_k set
+  OpBcTotal<MAG, RAD> t;
+  KD_HD long size() const { return e.size() + t.size(); }
+  KD_INL void operator()(long i) const
+  {
+    if (i < e.size())
+      e(i);
+    else
+      t(i - e.size());
   }
 };
 
@@ -1141,14 +1162,14 @@ void set_bc(Ex &ex, const GridBufs &s)
     ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u});
     auto edges = [&](int a, int b) {
       EdgeGeo<M, R> E{G, a, b};
-      ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u});
+      ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u, false});
     };
     if (G.ny > 1) {
       edges(0, 1);
       if (G.nz > 1) {
         edges(0, 2);
-        edges(1, 2);
-        ex.each(OpBcTotal<M, R>{s.P, G, s.p, s.u});
+        EdgeGeo<M, R> E{G, 1, 2};
+        ex.each(OpBcEdgeTotal<M, R>{OpBcEdgeA<M, R>{s.P, E, s.p, s.u, true}, OpBcTotal<M, R>{s.P, G, s.p, s.u}});
       }
     }
   });
