@@ -1021,10 +1021,13 @@ struct OpCombSeq {
   CombStmt st[3];
   const double *clk;
   double gamma;
+  unsigned *zero = nullptr;  // counters cleared by the first nzero items
+  int nzero = 0;
   // one interior cell per item, all its variables
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   KD_INL void operator()(long i) const
   {
+    if (i < nzero) zero[i] = 0u;
     const double dt = clk[1];
     const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
     const long c = cidx(G, ix, iy, iz);
@@ -1189,7 +1192,7 @@ void tstepden(Ex &ex, const GridBufs &s)
 // the current p and u (the ghost fill is a function of the interior alone,
 // so a second fill would write the same values) and are not refilled.
 template <class Ex>
-void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false)
+void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false, bool counts_cleared = false)
 {
   const DGrid &G = s.G;
   const koral_params &P = s.P;
@@ -1197,7 +1200,7 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false)
   tstepden(ex, s);
   const int tt = P.mag ? 1 : 0;
   const int n[3] = {G.nx, G.ny, G.nz};
-  ex.each(OpZeroU32{s.fcnt, 3});
+  if (!counts_cleared) ex.each(OpZeroU32{s.fcnt, 3});
   // x2 first: its floor pass (faces next to the magnetised torus surface,
   // the longest) then overlaps the x1 and x3 sweeps.  The sweeps write
   // separate flux arrays, so their order does not change any result.
@@ -1272,11 +1275,11 @@ void fixup(Ex &ex, const GridBufs &s, int type)
 
 // calc_u2p with its cell_fixup passes and the ghost refresh
 template <class Ex>
-void u2p(Ex &ex, const GridBufs &s)
+void u2p(Ex &ex, const GridBufs &s, bool count_cleared = false)
 {
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
-    ex.each(OpZeroU32{s.fcnt + 3, 1});
+    if (!count_cleared) ex.each(OpZeroU32{s.fcnt + 3, 1});
     const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy};
     ex.count(op, s.counters + 3);
     ex.count(OpU2PFloor<T::MAG, T::RAD>{op}, s.counters + 3);
@@ -1393,39 +1396,48 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   auto A = [](Coef c, const double *x) { return CombTerm{c, x, 0}; };
   auto R = [](Coef c, int q) { return CombTerm{c, nullptr, q}; };
   const CombTerm none{{Coef::CONST, 0.}, nullptr, 0};
+  // each seq() precedes a u2p and also clears the four list counters of the
+  // u2p and of the explicit operator that follows it (no reader of them runs
+  // in between: the floor passes of the previous sweeps were joined)
   auto seq = [&](std::initializer_list<CombStmt> l) {
     OpCombSeq op{s.G, s.P.nv, (int)l.size(), {}, s.clock, gamma};
     int q = 0;
     for (const CombStmt &st : l) op.st[q++] = st;
+    if (op.size() >= 4) {
+      op.zero = s.fcnt;
+      op.nzero = 4;
+    } else {
+      ex.each(OpZeroU32{s.fcnt, 4});
+    }
     ex.each(op);
   };
   comb(ex, s, gamma, s.ut0, one, s.u);
   implicit_op(ex, s, gamma);
   // drt1 = (u - ut0) / (gamma dt); ut1 = u
   seq({CombStmt{s.drt1, 2, {A(idtg, s.u), A(midtg, s.ut0), none}}, CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
-  u2p(ex, s);
+  u2p(ex, s, true);
   do_correct(ex, s);
   // u2p ended with set_bc; without a polar correction p and u are unchanged
-  explicit_op(ex, s, !polar_correction_active(s));
+  explicit_op(ex, s, !polar_correction_active(s), true);
   // dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
   seq({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
        CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
        CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
-  u2p(ex, s);
+  u2p(ex, s, true);
   do_correct(ex, s);
   implicit_op(ex, s, gamma);
   // drt2 = (u - uforget) / (gamma dt); ut2 = u
   seq({CombStmt{s.drt2, 2, {A(idtg, s.u), A(midtg, s.uforget), none}},
        CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
-  u2p(ex, s);
+  u2p(ex, s, true);
   do_correct(ex, s);
   // u2p ended with set_bc; without a polar correction p and u are unchanged
-  explicit_op(ex, s, !polar_correction_active(s));
+  explicit_op(ex, s, !polar_correction_active(s), true);
   // dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
   seq({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
        CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
        CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
-  u2p(ex, s);
+  u2p(ex, s, true);
   do_correct(ex, s);
 }
 
