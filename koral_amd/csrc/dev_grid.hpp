@@ -22,25 +22,37 @@ struct DGrid {
   int bc[6];
 };
 
+// Index functions of the ghost-inclusive box.  A collapsed dimension (n = 1)
+// ignores its coordinate.  Cells span [-ng_d, n_d + ng_d) per dimension,
+// faces of direction d one more along d.
+KD_HD bool in_cells(int i, int n, int ng) { return n == 1 || (i >= -ng && i < n + ng); }
+KD_HD bool in_faces(int i, int n, int ng) { return i >= -ng && i <= n + ng; }
+
 KD_HD long cidx(const DGrid &G, int ix, int iy, int iz)
 {
+  KD_CHECK(in_cells(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy) && in_cells(iz, G.nz, G.ngz), "cidx", ix, iy, iz);
   long y = G.ny > 1 ? iy + G.ngy : 0;
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * G.sy + y) * G.sx + ix + G.ngx;
 }
 KD_INL long fxidx(const DGrid &G, int ix, int iy, int iz)
 {
+  KD_CHECK(in_faces(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy) && in_cells(iz, G.nz, G.ngz), "fxidx", ix, iy, iz);
   long y = G.ny > 1 ? iy + G.ngy : 0;
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * G.sy + y) * (G.sx + 1) + ix + G.ngx;
 }
 KD_INL long fyidx(const DGrid &G, int ix, int iy, int iz)
 {
+  KD_CHECK(G.ny > 1 && in_cells(ix, G.nx, G.ngx) && in_faces(iy, G.ny, G.ngy) && in_cells(iz, G.nz, G.ngz), "fyidx",
+           ix, iy, iz);
   long z = G.nz > 1 ? iz + G.ngz : 0;
   return (z * (G.sy + 1) + iy + G.ngy) * G.sx + ix + G.ngx;
 }
 KD_INL long fzidx(const DGrid &G, int ix, int iy, int iz)
 {
+  KD_CHECK(G.nz > 1 && in_cells(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy) && in_faces(iz, G.nz, G.ngz), "fzidx",
+           ix, iy, iz);
   long y = G.ny > 1 ? iy + G.ngy : 0;
   return ((long)(iz + G.ngz) * G.sy + y) * G.sx + ix + G.ngx;
 }
@@ -50,12 +62,16 @@ KD_INL long fidx(const DGrid &G, int d, int ix, int iy, int iz)
 }
 KD_INL long midx(const DGrid &G, int ix, int iy)
 {
+  KD_CHECK(in_cells(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy), "midx", ix, iy, 0);
   return (long)(G.ny > 1 ? iy + G.ngy : 0) * G.sx + ix + G.ngx;
 }
 KD_HD int dimn(const DGrid &G, int d) { return d == 0 ? G.nx : (d == 1 ? G.ny : G.nz); }
 
+// width of cell ic along d; the face coordinates xb_d hold n_d + 2 NG + 1
+// entries in every dimension, collapsed ones included
 KD_INL double size_x(const DGrid &G, int ic, int d)
 {
+  KD_CHECK(d >= 0 && d < 3 && ic >= -G.ng && ic < dimn(G, d) + G.ng, "size_x", ic, d, 0);
   const double *xb = d == 0 ? G.xb0 : (d == 1 ? G.xb1 : G.xb2);
   return xb[ic + 1 + G.ng] - xb[ic + G.ng];
 }
@@ -69,9 +85,11 @@ KD_INL void geom_cell(const DGrid &G, int ix, int iy, Geom &g)
 KD_INL void geom_face(const DGrid &G, int d, int ix, int iy, Geom &g)
 {
   if (d == 0) {
+    KD_CHECK(in_faces(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy), "geom_face x", ix, iy, 0);
     const long m = (long)(G.ny > 1 ? iy + G.ngy : 0) * (G.sx + 1) + ix + G.ngx, n = G.nplx;
     load_geom(g, [&](int c) { return G.gfx[c * n + m]; }, [&](int c) { return G.Gfx[c * n + m]; });
   } else if (d == 1) {
+    KD_CHECK(G.ny > 1 && in_cells(ix, G.nx, G.ngx) && in_faces(iy, G.ny, G.ngy), "geom_face y", ix, iy, 0);
     const long m = (long)(iy + G.ngy) * G.sx + ix + G.ngx, n = G.nply;
     load_geom(g, [&](int c) { return G.gfy[c * n + m]; }, [&](int c) { return G.Gfy[c * n + m]; });
   } else {

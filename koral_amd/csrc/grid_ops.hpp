@@ -1101,6 +1101,50 @@ inline long face_states_cap(const DGrid &G)
   return w(G.nx) * w(G.ny) * w(G.nz);
 }
 
+// Every device buffer of a grid state with its element count: f(ptr, count,
+// name) for each, ptr a reference to the GridBufs member.  The library
+// allocates exactly these (koral_hip_grid_create); the host check of the
+// operators allocates the same sizes behind guard pages.  Directions that are
+// collapsed get no flux, face-state or floor-list buffers.
+constexpr int IMP_LIST_COUNTERS = 32;
+template <class F>
+void grid_buffers(GridBufs &s, F &&f)
+{
+  const DGrid &G = s.G;
+  const long nv = s.P.nv, nc = G.ncell, nd = (long)G.nx * G.ny * G.nz;
+  const long nf[3] = {G.nfx, G.nfy, G.nfz};
+  f(s.p, nv * nc, "p");
+  f(s.u, nv * nc, "u");
+  f(s.fl, (long)NFLAGS * nc, "flags");
+  f(s.ws, (long)NWS * nc, "wavespeeds");
+  for (int d = 0; d < 3; d++)
+    if (dimn(G, d) > 1) {
+      f(s.flb[d], nv * nf[d], d == 0 ? "flux x" : d == 1 ? "flux y" : "flux z");
+      f(s.fs[d], 2 * nv * face_states_cap(G), d == 0 ? "face states x" : d == 1 ? "face states y" : "face states z");
+      f(s.flist[d], face_states_cap(G), d == 0 ? "floor list x" : d == 1 ? "floor list y" : "floor list z");
+    }
+  f(s.fcnt, 4L, "list counts");
+  f(s.ulist, nd, "u2p list");
+  f(s.uwhy, nd, "u2p list kinds");
+  f(s.emf, 3L * (G.nx + 1) * (G.ny + 1) * (G.nz + 1), "emf");
+  f(s.pn, nv * nc, "fixup p");
+  f(s.un, nv * nc, "fixup u");
+  f(s.mark, nc, "fixup marks");
+  f(s.ut0, nv * nc, "ut0");
+  f(s.ut1, nv * nc, "ut1");
+  f(s.uforget, nv * nc, "uforget");
+  f(s.dut1, nv * nc, "dut1");
+  f(s.dut2, nv * nc, "dut2");
+  f(s.drt1, nv * nc, "drt1");
+  f(s.drt2, nv * nc, "drt2");
+  f(s.tsd, 1L, "tstepden");
+  f(s.clock, 4L, "clock");
+  f(s.counters, 8L, "counters");
+  for (int i = 0; i < 2; i++)
+    for (int k = 0; k < 2; k++) f(s.imp_list[i][k], nd, "implicit list");
+  f(s.imp_ctr, (long)IMP_LIST_COUNTERS, "implicit counters");
+}
+
 // Ghost-inclusive layout of a grid description (a collapsed dimension,
 // n = 1, has no ghosts).  Table pointers are left for the caller.
 inline void grid_layout(const koral_grid_desc *D, DGrid &G)

@@ -15,6 +15,8 @@
 
 using namespace kd;
 
+static_assert(IMP_NCTR == IMP_LIST_COUNTERS, "implicit pass counters");
+
 namespace {
 
 constexpr int BS = 256;
@@ -444,35 +446,10 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   up(D->gfy, 20 * G.nply, &G.gfy);
   up(D->Gfy, 20 * G.nply, &G.Gfy);
   const long nc = G.ncell;
-  if (!rc) rc = g->alloc(&s.p, nv * nc);
-  if (!rc) rc = g->alloc(&s.u, nv * nc);
-  if (!rc) rc = g->alloc(&s.fl, NFLAGS * nc);
-  if (!rc) rc = g->alloc(&s.ws, NWS * nc);
-  if (!rc && G.nx > 1) rc = g->alloc(&s.flb[0], nv * G.nfx);
-  if (!rc && G.ny > 1) rc = g->alloc(&s.flb[1], nv * G.nfy);
-  if (!rc && G.nz > 1) rc = g->alloc(&s.flb[2], nv * G.nfz);
-  for (int d = 0; d < 3 && !rc; d++)
-    if ((d == 0 ? G.nx : d == 1 ? G.ny : G.nz) > 1) {
-      rc = g->alloc(&s.fs[d], 2L * nv * face_states_cap(G));
-      if (!rc) rc = g->alloc(&s.flist[d], face_states_cap(G));
-    }
-  if (!rc) rc = g->alloc(&s.fcnt, 4);
-  if (!rc) rc = g->alloc(&s.ulist, (long)G.nx * G.ny * G.nz);
-  if (!rc) rc = g->alloc(&s.uwhy, (long)G.nx * G.ny * G.nz);
+  grid_buffers(s, [&](auto *&ptr, long count, const char *) {
+    if (!rc) rc = g->alloc(&ptr, (size_t)count);
+  });
   if (!rc) rc = g->make_side();
-  if (!rc) rc = g->alloc(&s.emf, 3L * (G.nx + 1) * (G.ny + 1) * (G.nz + 1));
-  if (!rc) rc = g->alloc(&s.pn, nv * nc);
-  if (!rc) rc = g->alloc(&s.un, nv * nc);
-  if (!rc) rc = g->alloc(&s.mark, nc);
-  double **work[7] = {&s.ut0, &s.ut1, &s.uforget, &s.dut1, &s.dut2, &s.drt1, &s.drt2};
-  for (int k = 0; k < 7 && !rc; k++) rc = g->alloc(work[k], nv * nc);
-  if (!rc) rc = g->alloc(&s.tsd, 1);
-  if (!rc) rc = g->alloc(&s.clock, 4);
-  if (!rc) rc = g->alloc(&s.counters, 8);
-  const long nd = (long)G.nx * G.ny * G.nz;
-  for (int i = 0; i < 2 && !rc; i++)
-    for (int k = 0; k < 2 && !rc; k++) rc = g->alloc(&s.imp_list[i][k], nd);
-  if (!rc) rc = g->alloc(&s.imp_ctr, IMP_NCTR);
   if (!rc) rc = g->alloc(&g->stage, nv * nc);
   if (!rc) rc = check_hip(hipMemset(s.p, 0, nv * nc * sizeof(double)), "memset");
   if (!rc) rc = check_hip(hipMemset(s.u, 0, nv * nc * sizeof(double)), "memset");
