@@ -6,13 +6,14 @@
 // vectors are small fixed-size arrays indexed with compile-time constants after
 // unrolling (template parameters MAG/RAD fix NV), so they live in VGPRs.
 // Arithmetic order follows the reference expressions literally and the code is
-// built with -ffp-contract=off, so results agree with a CPU FP64 build to
-// rounding of the transcendental functions.
+// built with -ffp-contract=off; the transcendentals are the portable ones of
+// kd_math.h, so a CPU build of the same arithmetic returns the same bits.
 #pragma once
 #include "dev_qual.hpp"
 #include <math.h>
 #include <float.h>
 #include "koral_hip.h"
+#include "kd_math.h"
 
 namespace kd {
 
@@ -25,8 +26,9 @@ namespace kd {
 inline long append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
 {
   if (!pred) return -1;
-  list[*len] = v;
-  return (long)(*len)++;
+  const unsigned slot = __atomic_fetch_add(len, 1u, __ATOMIC_RELAXED);
+  list[slot] = v;
+  return (long)slot;
 }
 #else
 __device__ __forceinline__ long append_index(bool pred, unsigned v, unsigned *list, unsigned *len)
@@ -176,7 +178,7 @@ KD_INL double Sfromu(const koral_params &P, double rho, double u)
 {
   double gm1 = P.gamma - 1.;
   double n = 1.0 / gm1;
-  return rho * log(pow(gm1 * u, n) / pow(rho, n + 1.0));
+  return rho * kd_log(kd_pow(gm1 * u, n) / kd_pow(rho, n + 1.0));
 }
 
 // p2u_mhd, p2u.c:56-159
@@ -284,8 +286,8 @@ KD_INL void f_u2p(const koral_params &P, int etype, double Wp, const U2PCons &c,
     double rho0 = D / gamma;
     double pressure = (GM1 / G) * wmrho0;
     double indexn = 1.0 / GM1;
-    double inside = pow(pressure, indexn) / pow(rho0, indexn + 1.0);
-    double Ss = log(inside);
+    double inside = kd_pow(pressure, indexn) / kd_pow(rho0, indexn + 1.0);
+    double Ss = kd_log(inside);
     f = -c.Sc + D * Ss;
     err = fabs(f) / (fabs(c.Sc) + fabs(D * Ss));
     double dSsdrho = G / ((1.0 - G) * rho0);
@@ -583,7 +585,7 @@ KD_FUN int u2p_rad(const koral_params &P, const double *uu, double *pp, const Ge
     u2 = alpha * (Avcon[2] + 1. / 3. * Erf * g.G[0][2] * (4.0 * gammarel2 - 1.0)) / (4. / 3. * Erf * grel);
     u3 = alpha * (Avcon[3] + 1. / 3. * Erf * g.G[0][3] * (4.0 * gammarel2 - 1.0)) / (4. / 3. * Erf * grel);
   } else {
-    double g2slow = pow(1.0 + 10.0 * DBL_EPSILON, 2.0);
+    double g2slow = kd_pow(1.0 + 10.0 * DBL_EPSILON, 2.0);
     double g2fast = gmax * gmax;
     double As, Es, us[4] = {0., 0., 0., 0.}, Af, Ef, uf[4] = {0., 0., 0., 0.};
     m1_cold(g, Avcov, g2slow, As, Es, us);
@@ -656,7 +658,7 @@ KD_INL void calc_Gi(const koral_params &P, const double *pp, const Geom &g, doub
   double rho = pp[0], u = pp[1];
   double p = (P.gamma - 1.) * u;
   double Tgas = p * P.mu_gas * P.m_proton / P.k_boltz / rho;
-  double B = P.sigma_rad * pow(Tgas, 4.) / PI_KORAL;
+  double B = P.sigma_rad * kd_pow(Tgas, 4.) / PI_KORAL;
   double kappa = kappa_abs(P, rho, u);
   double kes = kappa_es(P, rho);
   double Ruu = 0.;

@@ -6,6 +6,7 @@
 #include <float.h>
 #include <stddef.h>
 #include "oracle.h"
+#include "o_math.h"
 
 #define B1(P) 6
 #define EE(P) (6 + 3 * (P)->mag)
@@ -106,7 +107,7 @@ double o_calc_Sfromu(const oracle_params *P, double rho, double u)
 {
   double gm1 = P->gamma - 1.;
   double n = 1.0 / gm1;
-  return rho * log(pow(gm1 * u, n) / pow(rho, n + 1.0));
+  return rho * O_LOG(O_POW(gm1 * u, n) / O_POW(rho, n + 1.0));
 }
 
 /* p2u_mhd: p2u.c:56-159 */
@@ -226,8 +227,8 @@ static void f_u2p_entropy(const oracle_params *P, double Wp, const double *c, do
   /* compute_specificentropy_wmrho0_idealgas, u2p.c:832-861 */
   double pressure = (GM1 / G) * wmrho0;
   double indexn = 1.0 / GM1;
-  double inside = pow(pressure, indexn) / pow(rho0, indexn + 1.0);
-  double Ss = log(inside);
+  double inside = O_POW(pressure, indexn) / O_POW(rho0, indexn + 1.0);
+  double Ss = O_LOG(inside);
   *f = -Sc + D * Ss;
   *err = fabs(*f) / (fabs(Sc) + fabs(D * Ss));
   double dSsdrho = G / ((1.0 - G) * rho0);

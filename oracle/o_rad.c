@@ -6,6 +6,7 @@
 #include <float.h>
 #include <string.h>
 #include "oracle.h"
+#include "o_math.h"
 
 #define SMALL 1.e-50
 #define BIG (1. / SMALL)
@@ -61,7 +62,7 @@ void o_calc_Gi(const oracle_params *P, const double *pp, const o_geom *g, double
   double rho = pp[0], u = pp[1];
   double p = (P->gamma - 1.) * u;
   double Tgas = p * P->mu_gas * P->m_proton / P->k_boltz / rho;
-  double B = P->sigma_rad * pow(Tgas, 4.) / PI_KORAL;
+  double B = P->sigma_rad * O_POW(Tgas, 4.) / PI_KORAL;
   double kappa = o_kappa_abs(P, pp);
   double kes = o_kappa_es(P, pp);
   double Ruu = 0.;
@@ -182,7 +183,7 @@ int o_u2p_rad(const oracle_params *P, const double *uu, double *pp, const o_geom
       Avs[j] = Avf[j] = Avcov[j];
       urs[j] = urff[j] = 0.;
     }
-    double g2slow = pow(1.0 + 10.0 * DBL_EPSILON, 2.0);
+    double g2slow = O_POW(1.0 + 10.0 * DBL_EPSILON, 2.0);
     double g2fast = gmax * gmax;
     m1_cold(g, Avs, Avsc, g2slow, &Es, urs);
     m1_cold(g, Avf, Avfc, g2fast, &Ef, urff);
