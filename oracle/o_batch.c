@@ -201,6 +201,11 @@ int oracle_grid_explicit_parts(const oracle_params *P, const o_grid *G, double t
   return o_op_explicit_parts(P, G, t, dt, u, p, tstepdenmax, ws, fl);
 }
 
+int oracle_grid_cell_fixup(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int type)
+{
+  return o_cell_fixup(P, G, u, p, flags, type);
+}
+
 int oracle_grid_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts)
 {
   return o_calc_u2p(P, G, u, p, flags, counts);

@@ -886,6 +886,13 @@ static void cell_fixup(const oracle_params *P, const o_grid *G, double *u, doubl
   free(un);
 }
 
+/* cell_fixup on its own (test entry of the oracle) */
+int o_cell_fixup(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int type)
+{
+  cell_fixup(P, G, u, p, flags, type);
+  return 0;
+}
+
 /* calc_u2p: finite.c:502-561 with calc_primitives u2p.c:13-133 */
 int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts)
 {

@@ -25,6 +25,7 @@ int o_op_explicit(const oracle_params *P, const o_grid *G, double t, double dt, 
                   double *tstepdenmax);
 int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
                         double *tstepdenmax, double *ws_out, double **flb_out);
+int o_cell_fixup(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int type);
 int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts);
 int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                   int *counts);

@@ -226,6 +226,7 @@ def _grid_lib():
         L.oracle_grid_explicit_parts.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
                                                  _dp, _dp, _dp]
         L.oracle_grid_u2p.argtypes = [vp, vp, _dp, _dp, _ip, _ip]
+        L.oracle_grid_cell_fixup.argtypes = [vp, vp, _dp, _dp, _ip, ctypes.c_int]
         L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
         L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
         L.oracle_grid_do_correct.argtypes = [vp, vp, _dp, _dp]
@@ -282,6 +283,12 @@ class GridState:
         _grid_lib().oracle_grid_u2p(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p),
                                     _i(self.flags), _i(c))
         return c
+
+    def cell_fixup(self, ftype):
+        """cell_fixup (finite.c:4468-4640) of flag index ftype (1 RADIMPFIXUP,
+        2 HDFIXUP, 3 RADFIXUP) on the current state and flags."""
+        _grid_lib().oracle_grid_cell_fixup(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p),
+                                           _i(self.flags), int(ftype))
 
     def do_correct(self):
         """do_correct: the polar-axis correction when P.polar_nc > 0."""
