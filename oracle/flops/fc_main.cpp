@@ -1,5 +1,7 @@
 // Average operation counts per implicit solve and per u2p inversion of the
-// oracle on a batch read from a file written by koral_amd (seeded states).
+// oracle on a batch read from a file written by koral_amd (seeded states),
+// and per zone-update of each operator of one RK2IMEX grid step on a grid
+// set-up read from a file in the host check's input format.
 #include <stdio.h>
 #include "cd.hpp"
 #define double cd
@@ -8,12 +10,14 @@
 FlopCount g_flops;
 
 static int rd(FILE *f, void *p, size_t n) { return fread(p, 1, n, f) == n ? 0 : -1; }
+int fc_grid_counts(FILE *f);
 
 int main(int argc, char **argv)
 {
-  if (argc < 3) { fprintf(stderr, "usage: flopcount implicit|u2p batch.bin\n"); return 2; }
+  if (argc < 3) { fprintf(stderr, "usage: flopcount implicit|u2p batch.bin | grid input.bin\n"); return 2; }
   FILE *f = fopen(argv[2], "rb");
   if (!f) return 2;
+  if (argv[1][0] == 'g') return fc_grid_counts(f);
   oracle_params P;
   long n = 0;
   if (rd(f, &P, sizeof P) || rd(f, &n, sizeof n)) return 2;

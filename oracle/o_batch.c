@@ -227,3 +227,10 @@ int oracle_grid_step(const oracle_params *P, const o_grid *G, double t, double d
 {
   return o_step_rk2imex(P, G, t, dt, u, p, flags, tstepdenmax, counts);
 }
+
+/* threads of the grid operators' independent loops (1: serial) */
+int oracle_grid_set_threads(int n)
+{
+  o_grid_set_threads(n);
+  return 0;
+}

@@ -2,8 +2,13 @@
  * Oracle: grid-level operators of one KORAL timestep -- ghost fill, wave
  * speeds, reconstruction, fluxes, Riemann solver, flux-CT, conservative update
  * with metric sources, the u2p sweep with fixups, the implicit sweep, and the
- * RK2IMEX step body.  Single-threaded restatement on AoS arrays laid out as
- * KORAL's (ko.h:505-520): state[(iz*SY + iy)*SX + ix][iv] with ghosts.
+ * RK2IMEX step body, restated on AoS arrays laid out as KORAL's
+ * (ko.h:505-520): state[(iz*SY + iy)*SX + ix][iv] with ghosts.
+ * Loops whose iterations each write only their own cell or face (wave speeds,
+ * face sweeps, update, u2p, implicit, the stage algebra) may run on several
+ * OpenMP threads (o_grid_threads, 1 by default, as KORAL's per-rank loops
+ * run); results do not depend on the thread count.  The ghost fill and
+ * cell_fixup, whose order matters, stay serial.
  * Test infrastructure only (see oracle.h).
  */
 #include <math.h>
@@ -14,6 +19,10 @@
 
 #define SMALL 1.e-50
 #define BIG (1. / SMALL)
+
+/* threads of the independent loops (set by oracle_grid_set_threads) */
+static int o_nthreads = 1;
+void o_grid_set_threads(int n) { o_nthreads = n < 1 ? 1 : n; }
 
 static inline double mx(double a, double b) { return a > b ? a : b; }
 static inline double mn(double a, double b) { return a < b ? a : b; } /* misc.c:56-60 */
@@ -492,6 +501,7 @@ static double calc_wavespeeds(const oracle_params *P, const o_grid *G, const dou
   int lx = lim, ly = G->ny > 1 ? lim : 0, lz = G->nz > 1 ? lim : 0;
   int corners = P->mag; /* MPI4CORNERS is forced by MAGNFIELD (choices.h:546-567) */
   double tmax = -1.;
+#pragma omp parallel for collapse(3) schedule(dynamic, 64) reduction(max : tmax) num_threads(o_nthreads)
   for (int ix = -lx; ix < G->nx + lx; ix++)
     for (int iy = -ly; iy < G->ny + ly; iy++)
       for (int iz = -lz; iz < G->nz + lz; iz++) {
@@ -569,6 +579,7 @@ static void sweep(const oracle_params *P, const o_grid *G, const double *p, cons
   double *pbL = calloc(nfaces * nv, sizeof(double)), *pbR = calloc(nfaces * nv, sizeof(double));
   double *flL = calloc(nfaces * nv, sizeof(double)), *flR = calloc(nfaces * nv, sizeof(double));
   (void)NG;
+#pragma omp parallel for collapse(3) schedule(dynamic, 64) num_threads(o_nthreads)
   for (int ix = -l[0]; ix < n[0] + l[0]; ix++)
     for (int iy = -l[1]; iy < n[1] + l[1]; iy++)
       for (int iz = -l[2]; iz < n[2] + l[2]; iz++) {
@@ -628,6 +639,7 @@ static void sweep(const oracle_params *P, const o_grid *G, const double *p, cons
     else if (n[k] == 1) { lo[k] = 0; hi[k] = 1; }
     else { lo[k] = -t; hi[k] = n[k] + t; }
   }
+#pragma omp parallel for collapse(3) schedule(dynamic, 64) num_threads(o_nthreads)
   for (int ix = lo[0]; ix < hi[0]; ix++)
     for (int iy = lo[1]; iy < hi[1]; iy++)
       for (int iz = lo[2]; iz < hi[2]; iz++) {
@@ -787,6 +799,7 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
     sweep(P, G, p, ws, d, flb[d]);
   }
   if (P->mag) flux_ct(P, G, flb[0], flb[1], flb[2]);
+#pragma omp parallel for collapse(3) schedule(dynamic, 64) num_threads(o_nthreads)
   for (int ix = 0; ix < G->nx; ix++)
     for (int iy = 0; iy < G->ny; iy++)
       for (int iz = 0; iz < G->nz; iz++) {
@@ -897,6 +910,8 @@ int o_cell_fixup(const oracle_params *P, const o_grid *G, double *u, double *p, 
 int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts)
 {
   int nv = P->nv;
+  int c0 = 0, c1 = 0, c2 = 0;
+#pragma omp parallel for collapse(3) schedule(dynamic, 64) reduction(+ : c0, c1, c2) num_threads(o_nthreads)
   for (int ix = 0; ix < G->nx; ix++)
     for (int iy = 0; iy < G->ny; iy++)
       for (int iz = 0; iz < G->nz; iz++) {
@@ -916,12 +931,15 @@ int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, in
         o_p2u(P, pp, uu, &g);
         fl[O_HDFIXUPFLAG] = fixups[0] > 0 ? 1 : 0;
         fl[O_RADFIXUPFLAG] = fixups[1] > 0 ? -1 : 0;
-        if (counts) {
-          counts[0] += fixups[0] > 0;
-          counts[1] += fixups[1] > 0;
-          counts[2] += corrected[0] == 1;
-        }
+        c0 += fixups[0] > 0;
+        c1 += fixups[1] > 0;
+        c2 += corrected[0] == 1;
       }
+  if (counts) {
+    counts[0] += c0;
+    counts[1] += c1;
+    counts[2] += c2;
+  }
   cell_fixup(P, G, u, p, flags, O_HDFIXUPFLAG);
   if (P->rad) cell_fixup(P, G, u, p, flags, O_RADFIXUPFLAG);
   o_set_bc(P, G, u, p);
@@ -934,6 +952,8 @@ int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, 
 {
   if (!P->rad) return 0;
   int nv = P->nv;
+  int c0 = 0, c1 = 0, c2 = 0;
+#pragma omp parallel for collapse(3) schedule(dynamic, 16) reduction(+ : c0, c1, c2) num_threads(o_nthreads)
   for (int ix = 0; ix < G->nx; ix++)
     for (int iy = 0; iy < G->ny; iy++)
       for (int iz = 0; iz < G->nz; iz++) {
@@ -942,12 +962,15 @@ int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, 
         o_geom_cell(G, ix, iy, &g);
         o_imp_info inf;
         int r = o_solve_implicit_lab(P, u + (long)nv * c, p + (long)nv * c, &g, dt, t, flags + O_NFLAGS * c, &inf);
-        if (counts) {
-          counts[0] += (r == 0);
-          counts[1] += inf.iters;
-          counts[2] += (r != 0);
-        }
+        c0 += (r == 0);
+        c1 += inf.iters;
+        c2 += (r != 0);
       }
+  if (counts) {
+    counts[0] += c0;
+    counts[1] += c1;
+    counts[2] += c2;
+  }
   cell_fixup(P, G, u, p, flags, O_RADIMPFIXUPFLAG);
   return 0;
 }
@@ -1023,7 +1046,9 @@ int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt,
   double *ut0 = malloc(bytes), *ut1 = malloc(bytes), *ut2 = malloc(bytes), *uforget = malloc(bytes);
   double *dut1 = malloc(bytes), *dut2 = malloc(bytes), *drt1 = malloc(bytes), *drt2 = malloc(bytes);
   double tmax = -1.;
-#define DOM for (int ix = 0; ix < G->nx; ix++) for (int iy = 0; iy < G->ny; iy++) for (int iz = 0; iz < G->nz; iz++)
+#define DOM                                                                \
+  _Pragma("omp parallel for collapse(3) schedule(static) num_threads(o_nthreads)") \
+  for (int ix = 0; ix < G->nx; ix++) for (int iy = 0; iy < G->ny; iy++) for (int iz = 0; iz < G->nz; iz++)
 #define C(a) a[(long)nv * cidx(G, ix, iy, iz) + iv]
   DOM for (int iv = 0; iv < nv; iv++) C(ut0) = 1. * C(u);
   o_op_implicit(P, G, t, dt * gamma, u, p, flags, counts);

@@ -33,4 +33,5 @@ int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt,
                    double *tstepdenmax, int *counts);
 double o_tstepden(const oracle_params *P, const o_grid *G, const double *p);
 int o_do_correct(const oracle_params *P, const o_grid *G, double *u, double *p);
+void o_grid_set_threads(int n);
 #endif

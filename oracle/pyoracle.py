@@ -230,8 +230,15 @@ def _grid_lib():
         L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
         L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
         L.oracle_grid_do_correct.argtypes = [vp, vp, _dp, _dp]
+        L.oracle_grid_set_threads.argtypes = [ctypes.c_int]
         L._grid_sigs = True
     return L
+
+
+def grid_set_threads(n):
+    """OpenMP threads of the grid operators' independent loops (1: serial;
+    results do not depend on it)."""
+    _grid_lib().oracle_grid_set_threads(int(n))
 
 
 class GridState:

@@ -209,3 +209,33 @@ def test_polar_axis_correction(oracle, rad):
     keep = [iv for iv in range(P.nv) if not 6 <= iv <= 8]
     assert np.array_equal(S.u[touched][:, keep], uu[:, keep])
     assert np.array_equal(S.p[~touched], p0[~touched]) and np.array_equal(S.u[~touched], u0[~touched])
+
+
+@pytest.mark.parametrize("kind", ["torus_mhd", "torus_rad"])
+def test_threaded_step_is_bit_identical(oracle, kind):
+    """The oracle's per-cell loops (wave speeds, sweeps, update, u2p, implicit,
+    stage algebra) on several threads give the serial result bit for bit:
+    two RK2IMEX steps, state, flags, counters and the next time step."""
+    if kind == "torus_mhd":
+        P = KP.params_torus_mhd()
+        spec, p = PR.torus_setup(P, n=(16, 16, 8))
+    else:
+        P = KP.params_torus_rad()
+        spec, p = PR.torus_setup(P, n=(12, 12, 8))
+    runs = []
+    try:
+        for threads in (1, 4):
+            oracle.grid_set_threads(threads)
+            S = oracle.GridState(P, oracle.Grid(spec), GR.to_aos(p))
+            S.init_u()
+            S.set_bc()
+            dt = 1. / S.tstepden()
+            tm1, c1 = S.step(0., dt)
+            tm2, c2 = S.step(dt, 1. / tm1)
+            runs.append((S.p.copy(), S.u.copy(), S.flags.copy(), (tm1, tm2), (list(c1), list(c2))))
+    finally:
+        oracle.grid_set_threads(1)
+    a, b = runs
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert a[3] == b[3] and a[4] == b[4]
