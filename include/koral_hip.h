@@ -111,6 +111,21 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
                              const double *g20, const double *G20, const double *dt,
                              double global_time, int *flags, int *info, void *stream);
 
+/* koral_hip_implicit_batch without any allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same arguments, kernels and results as koral_hip_implicit_batch, but the
+   work lists and counters live in a fixed device array of the library (16 MiB
+   per GPU, held until the process ends), so n may not exceed
+   KORAL_HIP_NOALLOC_MAX_CELLS (larger batches return KORAL_E_ARG).  Every call
+   on one GPU shares that array: calls that overlap on one GPU (other streams
+   or threads) race and must be serialised by the caller; calls in stream
+   order are safe (each clears the counters it uses). */
+#define KORAL_HIP_NOALLOC_MAX_CELLS (1L << 20)
+int koral_hip_implicit_batch_noalloc(const koral_params *P, long n, double *uu, double *pp,
+                                     const double *g20, const double *G20, const double *dt,
+                                     double global_time, int *flags, int *info, void *stream);
+
 /* ---- learned initial guess of the implicit solve (no reference
    counterpart: KORAL starts the Newton solve from the old state) ----
 

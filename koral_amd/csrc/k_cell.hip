@@ -313,6 +313,37 @@ int release_set(ImpSet *s, hipStream_t st)
 
 inline dim3 grid_for(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
+// Work lists and counters of koral_hip_implicit_batch_noalloc: a file-scope
+// device array (each GPU holds its own copy from module load until the
+// process ends), laid out as an ImpSet of KORAL_HIP_NOALLOC_MAX_CELLS cells.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): it uses the address hipGetSymbolAddress
+// returns on each device, kept in scratch_dev.
+constexpr long NOALLOC_LISTS = 4L * KORAL_HIP_NOALLOC_MAX_CELLS;
+static __device__ __align__(256) unsigned scratch[NOALLOC_LISTS + IMP_NCTR];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned *scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device, looked up once per
+// device with that device current
+int scratch_on_device(unsigned **out)
+{
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_error(KORAL_E_STATE, "implicit_batch_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *ptr = nullptr;
+    rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    scratch_dev[dev] = (unsigned *)ptr;
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+
 }  // namespace
 
 #define KORAL_DISPATCH(P, KERNEL, ...)                                          \
@@ -443,6 +474,39 @@ int koral_hip_implicit_batch(const koral_params *P, long n, double *uu, double *
   if (rc) return rc;
   if (rc2) return rc2;
   return check_launch("koral_hip_implicit_batch");
+}
+
+int koral_hip_implicit_batch_noalloc(const koral_params *P, long n, double *uu, double *pp, const double *g20,
+                                     const double *G20, const double *dt, double global_time, int *flags, int *info,
+                                     void *stream)
+{
+  if (check_params(P) || n < 0) return KORAL_E_ARG;
+  if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
+  if (n > KORAL_HIP_NOALLOC_MAX_CELLS)
+    return set_error(KORAL_E_ARG, "implicit_batch_noalloc: batch larger than KORAL_HIP_NOALLOC_MAX_CELLS "
+                                  "(use koral_hip_implicit_batch)");
+  unsigned *ptr = nullptr;
+  int rc = scratch_on_device(&ptr);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  // the passes clear the counters with a kernel before they use them; list
+  // entries are written before they are read
+  ImpBufs b{};
+  b.list[0][0] = ptr;
+  b.list[0][1] = ptr + KORAL_HIP_NOALLOC_MAX_CELLS;
+  b.list[1][0] = ptr + 2L * KORAL_HIP_NOALLOC_MAX_CELLS;
+  b.list[1][1] = ptr + 3L * KORAL_HIP_NOALLOC_MAX_CELLS;
+  b.ctr = ptr + NOALLOC_LISTS;
+  if (P->mag) {
+    BatchCells<1, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<1>(*P, src, n, b, nullptr, s);
+  } else {
+    BatchCells<0, 1> src{n, uu, pp, g20, G20, dt, flags, info, global_time};
+    rc = implicit_passes<0>(*P, src, n, b, nullptr, s);
+  }
+  if (rc) return rc;
+  return check_launch("koral_hip_implicit_batch_noalloc");
 }
 
 int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, double *pp, const double *g20,
