@@ -283,13 +283,14 @@ def cpu_baseline_implicit(P, pp, g20, G20, dt, args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     ns = min(args.cpu_sample, len(dt))
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
     uu = pyoracle.p2u(P, pp[:ns], g20[:ns], G20[:ns])
     t = time.perf_counter()
     pyoracle.implicit(P, uu, pp[:ns], g20[:ns], G20[:ns], dt[:ns], nthreads=threads)
     el = time.perf_counter() - t
-    return {"value": ns / el, "unit": "implicit solves/s", "cores": threads, "kind": "port",
-            "sample": "first %d cells of the same seeded batch, C oracle (gcc -O2, OpenMP)" % ns}
+    return {"value": ns / el, "unit": "implicit solves/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": "first %d cells of the same seeded batch, C oracle (gcc -O3, OpenMP, %d threads)"
+                      % (ns, threads)}
 
 
 # Algorithmic HBM bytes per thread-unit of each grid operator, counted as the
@@ -394,33 +395,59 @@ def bench_grid(args, kind):
                      "algorithmic_bytes_per_launch": bytes_launch},
     }
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline_grid(P, kind, args)
+        res["cpu_baseline"] = cpu_baseline_grid(P, kind, args, n)
     return res
 
 
-def cpu_baseline_grid(P, kind, args):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(args):
+    # the GPU box's host share is 16 cores (whatever os.cpu_count() shows)
+    return args.cpu_threads or min(16, os.cpu_count() or 1)
+
+
+def cpu_baseline_grid(P, kind, args, n_default):
+    """The oracle's RK2IMEX step (the C restatement of ko.c:444-570 and the
+    operators it calls; gcc -O3, OpenMP over the per-cell loops, results
+    independent of the thread count) on the same torus set-up, timed for
+    about --cpu-seconds on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from koral_amd import problems as PR, grid as GR
-    n = tuple(args.cpu_grid)
+    n = tuple(args.cpu_grid) if args.cpu_grid else n_default
+    threads = cpu_threads(args)
     spec, p = PR.torus_setup(P, n=n, seed=args.seed)
     G = pyoracle.Grid(spec)
     S = pyoracle.GridState(P, G, GR.to_aos(p))
-    S.init_u()
-    S.set_bc()
-    dt = 1. / S.tstepden()
-    t, steps, el = 0., 0, 0.
-    while steps < 1 or (el < args.cpu_seconds and steps < 50):
-        t0 = time.perf_counter()
-        tm, _ = S.step(t, dt)
-        el += time.perf_counter() - t0
-        t += dt
-        dt = 1. / tm
-        steps += 1
+    pyoracle.grid_set_threads(threads)
+    try:
+        S.init_u()
+        S.set_bc()
+        dt = 1. / S.tstepden()
+        t, steps, el = 0., 0, 0.
+        while steps < 1 or (el < args.cpu_seconds and steps < 50):
+            t0 = time.perf_counter()
+            tm, _ = S.step(t, dt)
+            el += time.perf_counter() - t0
+            t += dt
+            dt = 1. / tm
+            steps += 1
+    finally:
+        pyoracle.grid_set_threads(1)
     zones = n[0] * n[1] * n[2]
-    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d RK2IMEX steps of the same torus set-up at %dx%dx%d, serial C oracle (gcc -O2)"
-                      % (steps, n[0], n[1], n[2])}
+    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": "%d RK2IMEX steps of the same torus set-up at %dx%dx%d, C oracle (gcc -O3, OpenMP, "
+                      "%d threads)" % (steps, n[0], n[1], n[2], threads)}
 
 
 def main():
@@ -430,8 +457,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="grmhd", choices=["implicit", "learned", "grmhd", "grrmhd"])
     ap.add_argument("--grid", type=int, nargs=3, default=None)
-    ap.add_argument("--cpu-grid", type=int, nargs=3, default=[48, 48, 24])
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-grid", type=int, nargs=3, default=None,
+                    help="grid of the CPU baseline (default: the benchmark's own grid)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cells", type=int, default=1 << 22)
     ap.add_argument("--train-cells", type=int, default=1 << 23)
     ap.add_argument("--epochs", type=int, default=8)

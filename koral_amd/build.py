@@ -53,6 +53,24 @@ def build(verbose=False, jobs=None):
     return OUT
 
 
+def build_binding(verbose=False):
+    """The KORAL-signature binding (koral_amd/binding/koral_ops.c, plain C as a
+    KORAL build would compile it) and the test driver that plays ko.c
+    (tests/host/binding_driver.c), built with gcc against include/koral_hip.h
+    and linked with libkoral_hip.so (found next to the driver by rpath)."""
+    out = os.path.join(ROOT, "tests", "host", "binding_driver")
+    cmd = ["gcc", "-std=gnu11", "-O2", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "binding"), os.path.join(HERE, "binding", "koral_ops.c"),
+           os.path.join(ROOT, "tests", "host", "binding_driver.c"), "-L", HERE, "-lkoral_hip",
+           "-Wl,-rpath,$ORIGIN/../../koral_amd", "-lm", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gcc failed for the binding driver:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

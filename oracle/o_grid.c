@@ -24,6 +24,25 @@
 static int o_nthreads = 1;
 void o_grid_set_threads(int n) { o_nthreads = n < 1 ? 1 : n; }
 
+/* Work arrays of the operators, kept between calls and grown to the largest
+   size asked for (KORAL keeps them as global arrays, ko.h:274-288; allocating
+   them per call would time page faults).  Contents are not cleared: every
+   entry an operator reads it has written in the same call.  Not re-entrant,
+   as KORAL's globals are not. */
+enum { W_PBL, W_PBR, W_FLL, W_FLR, W_WS, W_FLBX, W_FLBY, W_FLBZ, W_UT0, W_UT1, W_UT2, W_UFORGET, W_DUT1, W_DUT2,
+       W_DRT1, W_DRT2, W_PN, W_UN, W_EMF, W_NSLOTS };
+static double *o_wbuf[W_NSLOTS];
+static size_t o_wcap[W_NSLOTS];
+static double *o_work(int slot, size_t n)
+{
+  if (n > o_wcap[slot]) {
+    free(o_wbuf[slot]);
+    o_wbuf[slot] = malloc(sizeof(double) * n);
+    o_wcap[slot] = o_wbuf[slot] ? n : 0;
+  }
+  return o_wbuf[slot];
+}
+
 static inline double mx(double a, double b) { return a > b ? a : b; }
 static inline double mn(double a, double b) { return a < b ? a : b; } /* misc.c:56-60 */
 static inline double sgn(double x) { return x > 0. ? 1. : (x < 0. ? -1. : 1.); }
@@ -187,7 +206,9 @@ void o_set_bc(const oracle_params *P, const o_grid *G, double *u, double *p)
 {
   int NG = G->ng;
   int x0 = -G->ngx, x1 = G->nx + G->ngx, y0 = -G->ngy, y1 = G->ny + G->ngy, z0 = -G->ngz, z1 = G->nz + G->ngz;
-  /* faces: loop_2 = ghost cells that are outside in exactly one dimension */
+  /* faces: loop_2 = ghost cells that are outside in exactly one dimension
+     (each reads interior cells only, so the order does not matter) */
+#pragma omp parallel for collapse(3) schedule(dynamic, 256) num_threads(o_nthreads)
   for (int ix = x0; ix < x1; ix++)
     for (int iy = y0; iy < y1; iy++)
       for (int iz = z0; iz < z1; iz++) {
@@ -576,8 +597,8 @@ static void sweep(const oracle_params *P, const o_grid *G, const double *p, cons
   int t = P->mag ? 1 : 0; /* MPI4CORNERS: transverse rows -1..N */
   long nfaces = (d == 0) ? (long)(G->sx + 1) * G->sy * G->sz
                          : (d == 1 ? (long)G->sx * (G->sy + 1) * G->sz : (long)G->sx * G->sy * (G->sz + 1));
-  double *pbL = calloc(nfaces * nv, sizeof(double)), *pbR = calloc(nfaces * nv, sizeof(double));
-  double *flL = calloc(nfaces * nv, sizeof(double)), *flR = calloc(nfaces * nv, sizeof(double));
+  double *pbL = o_work(W_PBL, nfaces * nv), *pbR = o_work(W_PBR, nfaces * nv);
+  double *flL = o_work(W_FLL, nfaces * nv), *flR = o_work(W_FLR, nfaces * nv);
   (void)NG;
 #pragma omp parallel for collapse(3) schedule(dynamic, 64) num_threads(o_nthreads)
   for (int ix = -l[0]; ix < n[0] + l[0]; ix++)
@@ -651,7 +672,6 @@ static void sweep(const oracle_params *P, const o_grid *G, const double *p, cons
         o_geom_face(G, d, ix, iy, &g);
         face_pair_flux(P, G, d, pbL + f * nv, pbR + f * nv, flL + f * nv, flR + f * nv, wL, wR, &g, flb + f * nv);
       }
-  free(pbL); free(pbR); free(flL); free(flR);
 }
 
 /* flux_ct: magn.c:199-339 (Toth EMF averaging) */
@@ -662,7 +682,7 @@ static void flux_ct(const oracle_params *P, const o_grid *G, double *flbx, doubl
   double c2 = (NX > 1 && NZ > 1) ? 0.25 : 0.5;
   double c3 = (NX > 1 && NY > 1) ? 0.25 : 0.5;
   long ne = (long)(NX + 1) * (NY + 1) * (NZ + 1);
-  double *emf = calloc(3 * ne, sizeof(double));
+  double *emf = o_work(W_EMF, 3 * ne);
 #define EMF(k, ix, iy, iz) emf[(k - 1) + 3 * ((ix) + (long)(iy) * (NX + 1) + (long)(iz) * (NY + 1) * (NX + 1))]
 #define FX(iv, ix, iy, iz) flbx[fxidx(G, ix, NY > 1 ? iy : 0, NZ > 1 ? iz : 0) * nv + iv]
 #define FY(iv, ix, iy, iz) flby[fyidx(G, NX > 1 ? ix : 0, iy, NZ > 1 ? iz : 0) * nv + iv]
@@ -719,7 +739,6 @@ static void flux_ct(const oracle_params *P, const o_grid *G, double *flbx, doubl
 #undef FX
 #undef FY
 #undef FZ
-  free(emf);
 }
 
 /* f_metric_source_term_arb: physics.c:324-465 (GDETIN=1) */
@@ -786,7 +805,7 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
   (void)t;
   o_set_bc(P, G, u, p);
   long ncell = (long)G->sx * G->sy * G->sz;
-  double *ws = calloc(ncell * NWS, sizeof(double));
+  double *ws = o_work(W_WS, ncell * NWS);
   double tmax = calc_wavespeeds(P, G, p, ws, NULL);
   if (tstepdenmax && tmax > *tstepdenmax) *tstepdenmax = tmax;
   double *flb[3] = {NULL, NULL, NULL};
@@ -795,7 +814,7 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
   int n[3] = {G->nx, G->ny, G->nz};
   for (int d = 0; d < 3; d++) {
     if (n[d] == 1) continue;
-    flb[d] = calloc(nf[d] * nv, sizeof(double));
+    flb[d] = o_work(W_FLBX + d, nf[d] * nv);
     sweep(P, G, p, ws, d, flb[d]);
   }
   if (P->mag) flux_ct(P, G, flb[0], flb[1], flb[2]);
@@ -825,8 +844,6 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
   if (flb_out)
     for (int d = 0; d < 3; d++)
       if (flb[d] && flb_out[d]) memcpy(flb_out[d], flb[d], sizeof(double) * nf[d] * nv);
-  for (int d = 0; d < 3; d++) free(flb[d]);
-  free(ws);
   return 0;
 }
 
@@ -841,7 +858,7 @@ static void cell_fixup(const oracle_params *P, const o_grid *G, double *u, doubl
   if (type == O_RADIMPFIXUPFLAG && !P->doradimpfixups) return;
   int nv = P->nv, nvmhd = 6 + 3 * P->mag, ee = nvmhd;
   long ncell = (long)G->sx * G->sy * G->sz;
-  double *pn = malloc(sizeof(double) * nv * ncell), *un = malloc(sizeof(double) * nv * ncell);
+  double *pn = o_work(W_PN, nv * ncell), *un = o_work(W_UN, nv * ncell);
   memcpy(pn, p, sizeof(double) * nv * ncell);
   memcpy(un, u, sizeof(double) * nv * ncell);
   int NX = G->nx, NY = G->ny, NZ = G->nz;
@@ -895,8 +912,7 @@ static void cell_fixup(const oracle_params *P, const o_grid *G, double *u, doubl
           u[(long)nv * c + iv] = un[(long)nv * c + iv];
         }
       }
-  free(pn);
-  free(un);
+
 }
 
 /* cell_fixup on its own (test entry of the oracle) */
@@ -1043,8 +1059,10 @@ int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt,
   long ncell = (long)G->sx * G->sy * G->sz;
   size_t bytes = sizeof(double) * nv * ncell;
   double gamma = 1. - 1. / sqrt(2.);
-  double *ut0 = malloc(bytes), *ut1 = malloc(bytes), *ut2 = malloc(bytes), *uforget = malloc(bytes);
-  double *dut1 = malloc(bytes), *dut2 = malloc(bytes), *drt1 = malloc(bytes), *drt2 = malloc(bytes);
+  const size_t nw = bytes / sizeof(double);
+  double *ut0 = o_work(W_UT0, nw), *ut1 = o_work(W_UT1, nw), *ut2 = o_work(W_UT2, nw);
+  double *uforget = o_work(W_UFORGET, nw), *dut1 = o_work(W_DUT1, nw), *dut2 = o_work(W_DUT2, nw);
+  double *drt1 = o_work(W_DRT1, nw), *drt2 = o_work(W_DRT2, nw);
   double tmax = -1.;
 #define DOM                                                                \
   _Pragma("omp parallel for collapse(3) schedule(static) num_threads(o_nthreads)") \
@@ -1076,7 +1094,6 @@ int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt,
 #undef DOM
 #undef C
   if (tstepdenmax) *tstepdenmax = tmax;
-  free(ut0); free(ut1); free(ut2); free(uforget); free(dut1); free(dut2); free(drt1); free(drt2);
   return 0;
 }
 
