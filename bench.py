@@ -336,6 +336,20 @@ def bench_grid(args, kind):
     for _ in range(args.warmup):
         dg.step()
     zones = n[0] * n[1] * n[2]
+    # launch choices (block sizes, fused or split sweeps, deferred-pass
+    # mapping) searched on this state before the timed steps
+    tune_info = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        rows = dg.autotune(steps=args.tune_steps)
+        tn = dg.get_tune()
+        names = {k: dg.L.koral_hip_grid_op_name(k).decode() for k in range(32)}
+        chosen = [r for r in rows if r[0].startswith("chose")]
+        tune_info = {"seconds": time.perf_counter() - t0, "trials": len(rows) - len(chosen),
+                     "start_ms_per_step": rows[0][2], "end_ms_per_step": chosen[-1][2] if chosen else rows[0][2],
+                     "fused": list(tn.fused), "floor_wave": tn.floor_wave, "floor_nw": tn.floor_nw,
+                     "u2p_nw": tn.u2p_nw,
+                     "block": {names[k]: tn.block[k] for k in range(32) if names[k] and tn.block[k] != 256}}
     # pilot: three steps with HIP events around every launch give the
     # per-operator split and the dominant operator
     dg.timing(enable=True, read=False)
@@ -385,7 +399,7 @@ def bench_grid(args, kind):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded torus initial data)",
-        "config": {"workload": label, "grid": list(n), "nv": P.nv, "ng": spec.ng,
+        "config": {"workload": label, "grid": list(n), "nv": P.nv, "ng": spec.ng, "autotune": tune_info,
                    "counters_per_step": (counters / args.steps).tolist(),
                    "operator_ms_per_step_pilot": {k: v[0] / 3 for k, v in sorted(split.items(), key=lambda kv: -kv[1][0])}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -469,6 +483,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
+    ap.add_argument("--tune-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the single-GPU build holds the whole domain "

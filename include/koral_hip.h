@@ -309,6 +309,34 @@ int koral_hip_grid_read_restart(koral_hip_grid *g, const char *folder, int nout1
    when on, runs the steps without the graph. */
 int koral_hip_grid_use_graph(koral_hip_grid *g, int enable);
 
+/* ---- launch choices and their autotuner (no reference counterpart: KORAL's
+   CPU loops have no launch shape) ----
+
+   The choices of one grid: threads per block of each operator kind (index as
+   koral_hip_grid_op_name; 64, 128 or 256), per sweep direction whether the
+   reconstruction runs inside the flux kernel (fused = 1) or writes the face
+   states to HBM for a separate flux kernel (0), and the mapping of the
+   deferred passes (the few faces needing the b^2/rho floor re-inversion and
+   the cells needing the entropy inversion or that floor): one wave per face
+   or cell (floor_wave = 1, the inversion's trials one per lane) or one lane
+   per face or cell (0), with floor_nw / u2p_nw waves (or lanes) in flight.
+   Every choice computes the same results bit for bit. */
+typedef struct koral_grid_tune {
+  int block[32];
+  int fused[3];
+  int floor_wave;
+  long floor_nw, u2p_nw;
+} koral_grid_tune;
+int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t);
+int koral_hip_grid_set_tune(koral_hip_grid *g, const koral_grid_tune *t);
+/* Coordinate search over the choices above on the grid's current state at
+   time t with step dt: each candidate runs 1 + steps RK2IMEX steps from a
+   snapshot of the state (restored afterwards, so the grid ends as it began),
+   timed over the last `steps` with events; a change is kept when it is at
+   least 1 % faster.  The chosen set stays on the grid.  report (may be NULL)
+   receives one line per candidate: "name value ms_per_step". */
+int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, char *report, int report_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ namespace kd {
 // Operator kinds (per-kind launch timing and names in the C-ABI)
 enum OpKind {
   K_P2U, K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_FLUX, K_EMF, K_CT, K_UPDATE,
-  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_RECON, K_FLUX_FLOOR, K_U2P_FLOOR, K_NKINDS
+  K_U2P, K_FIXUP, K_FIXUP_APPLY, K_IMPLICIT, K_COMB, K_POLAR, K_RECON, K_FLUX_FLOOR, K_U2P_FLOOR, K_ZERO, K_NKINDS
 };
 
 struct Box {
@@ -573,11 +573,11 @@ struct OpFlux {
 };
 
 // The faces OpFlux left: the same face states, the full face floors, the
-// same fluxes, one wave per face (the floors' W inversions evaluate their
-// trials one per lane; lane 0 stores).  FLOOR_WAVES waves stride over the
-// list, whose length is known only on the device.
-constexpr long FLOOR_WAVES = 1024;
-template <int MAG, int RAD, int D>
+// same fluxes.  WAVE: one wave per face (the floors' W inversions evaluate
+// their trials one per lane; lane 0 stores); otherwise one lane per face.
+// nw waves (WAVE) or lanes stride over the list, whose length is known only
+// on the device.
+template <int MAG, int RAD, int D, bool WAVE = true>
 struct OpFluxFloor {
   static constexpr int kind = K_FLUX_FLOOR;
   static constexpr int waves_per_simd = 2;
@@ -587,13 +587,14 @@ struct OpFluxFloor {
   double *flb;
   const unsigned *list, *cnt;
   Box b;
-  KD_HD long size() const { return 64 * FLOOR_WAVES; }
+  long nw;
+  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
   KD_INL void operator()(long t) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     const long m = (long)*cnt;
-    const bool lead = (t & 63) == 0;
-    for (long j = t >> 6; j < m; j += FLOOR_WAVES) {
+    const bool lead = WAVE ? (t & 63) == 0 : true;
+    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
       const long i = list[j];
       int ix, iy, iz;
       b.at(i, ix, iy, iz);
@@ -601,8 +602,99 @@ struct OpFluxFloor {
       load_face_states<NV>(fs, b.size(), i, pL, pR);
       Geom g;
       geom_face(G, D, ix, iy, g);
-      floors_mhd<MAG, RAD, false, true>(P, pL, g);
-      floors_mhd<MAG, RAD, false, true>(P, pR, g);
+      floors_mhd<MAG, RAD, false, WAVE>(P, pL, g);
+      floors_mhd<MAG, RAD, false, WAVE>(P, pR, g);
+      face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g, lead);
+    }
+  }
+};
+
+// Both face states of the face (ix, iy, iz) of direction D reconstructed from
+// the cell values, variable by variable: the arithmetic of OpRecon, so the
+// states are the same bit for bit.
+template <int MAG, int RAD, int ORDER, int D>
+KD_INL void face_states(const koral_params &P, const DGrid &G, const double *__restrict__ p, int ix, int iy, int iz,
+                        double *pL, double *pR)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+  const long n = G.ncell, c0 = cidx(G, ix, iy, iz), st = dir_stride<D>(G);
+  double dx[6];
+  const int id = dir_index<D>(ix, iy, iz);
+#pragma unroll
+  for (int o = 0; o < 6; o++) dx[o] = (ORDER == 2) ? size_x(G, id + o - 3, D) : 0.;
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) {
+    recon_var<ORDER>(P, p + iv * n + c0, st, dx, pL[iv], pR[iv]);
+    // one variable's stencil in flight at a time
+    compiler_fence();
+  }
+}
+
+// One face sweep of direction D in a single kernel: reconstruction from the
+// cells (finite.c:694-1014), face floors (check_floors_mhd, finite.c:801-993),
+// fluxes and HLL/LF (f_calc_fluxes_at_faces finite.c:1297-1415).  The face
+// states never leave the registers.  A face whose states need the b^2/rho
+// floor is appended to list (cleared before the sweep) and left to
+// OpSweepFloor, which reconstructs it again.  Same results as OpRecon +
+// OpFlux bit for bit.
+template <int MAG, int RAD, int ORDER, int D>
+struct OpSweep {
+  static constexpr int kind = K_FLUX;
+  static constexpr int waves_per_simd = 2;
+  koral_params P;
+  DGrid G;
+  const double *p, *ws;
+  double *flb;
+  unsigned *list, *cnt;
+  Box b;
+  KD_HD long size() const { return b.size(); }
+  KD_INL void operator()(long i) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    int ix, iy, iz;
+    b.at(i, ix, iy, iz);
+    double pL[NV], pR[NV];
+    face_states<MAG, RAD, ORDER, D>(P, G, p, ix, iy, iz, pL, pR);
+    Geom g;
+    geom_face(G, D, ix, iy, g);
+    const int dl = floors_mhd<MAG, RAD, true>(P, pL, g);
+    const int dr = floors_mhd<MAG, RAD, true>(P, pR, g);
+    const bool defer = dl == FLOORS_DEFERRED || dr == FLOORS_DEFERRED;
+    append_index(defer, (unsigned)i, list, cnt);
+    if (defer) return;
+    face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g);
+  }
+};
+
+// The faces OpSweep left, reconstructed again from the cells, with the full
+// face floors; WAVE and nw as in OpFluxFloor.
+template <int MAG, int RAD, int ORDER, int D, bool WAVE = true>
+struct OpSweepFloor {
+  static constexpr int kind = K_FLUX_FLOOR;
+  static constexpr int waves_per_simd = 2;
+  koral_params P;
+  DGrid G;
+  const double *p, *ws;
+  double *flb;
+  const unsigned *list, *cnt;
+  Box b;
+  long nw;
+  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
+  KD_INL void operator()(long t) const
+  {
+    constexpr int NV = Lay<MAG, RAD>::NV;
+    const long m = (long)*cnt;
+    const bool lead = WAVE ? (t & 63) == 0 : true;
+    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
+      const long i = list[j];
+      int ix, iy, iz;
+      b.at(i, ix, iy, iz);
+      double pL[NV], pR[NV];
+      face_states<MAG, RAD, ORDER, D>(P, G, p, ix, iy, iz, pL, pR);
+      Geom g;
+      geom_face(G, D, ix, iy, g);
+      floors_mhd<MAG, RAD, false, WAVE>(P, pL, g);
+      floors_mhd<MAG, RAD, false, WAVE>(P, pR, g);
       face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g, lead);
     }
   }
@@ -610,7 +702,7 @@ struct OpFluxFloor {
 
 // n (<= 64) counters set to zero by a kernel (safe inside a captured graph)
 struct OpZeroU32 {
-  static constexpr int kind = K_FLUX_FLOOR;
+  static constexpr int kind = K_ZERO;
   unsigned *q;
   int n;
   KD_HD long size() const { return n; }
@@ -813,25 +905,26 @@ struct OpU2P {
   }
 };
 
-// the cells OpU2P left, with the full calc_primitives, one wave per cell
-// (the W inversions evaluate their trials one per lane); U2P_FLOOR_WAVES
-// waves stride over the list, whose length is known only on the device
-constexpr long U2P_FLOOR_WAVES = 4096;
-template <int MAG, int RAD>
+// the cells OpU2P left, with the full calc_primitives.  WAVE: one wave per
+// cell (the W inversions evaluate their trials one per lane), otherwise one
+// lane per cell; nw waves (or lanes) stride over the list, whose length is
+// known only on the device
+template <int MAG, int RAD, bool WAVE = true>
 struct OpU2PFloor {
   static constexpr int kind = K_U2P_FLOOR;
   static constexpr int waves_per_simd = 2;
   OpU2P<MAG, RAD> op;
-  KD_HD long size() const { return 64 * U2P_FLOOR_WAVES; }
+  long nw;  // waves (WAVE: one cell per wave) or lanes (one cell per lane) in flight
+  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
   KD_INL Cnt3 operator()(long t) const
   {
     Cnt3 sum{{0, 0, 0}};
     const long m = (long)*op.cnt;
-    const bool lead = (t & 63) == 0;
-    for (long j = t >> 6; j < m; j += U2P_FLOOR_WAVES) {
+    const bool lead = WAVE ? (t & 63) == 0 : true;
+    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
       const long i = (long)op.list[j];
-      const Cnt3 c = op.why[j] ? op.template cell<false, true, true>(i, nullptr, nullptr, lead)
-                                : op.template cell<false, false, true>(i, nullptr, nullptr, lead);
+      const Cnt3 c = op.why[j] ? op.template cell<false, true, WAVE>(i, nullptr, nullptr, lead)
+                                : op.template cell<false, false, WAVE>(i, nullptr, nullptr, lead);
 #pragma unroll
       for (int k = 0; k < 3; k++) sum.v[k] += c.v[k];
     }
@@ -1072,7 +1165,32 @@ struct OpCombSeq {
 // ---------------------------------------------------------------------------
 // Device-resident state of a grid: SoA primitives/conserved over the
 // ghost-inclusive box, int8 flags, and the work arrays of a step.
+// Launch choices of the grid operators, the space the autotuner searches
+// (koral_hip_grid_autotune).  Every choice computes the same results bit for
+// bit; they differ only in speed.
+constexpr int TUNE_KINDS = 32;  // koral_grid_tune.block entries (>= K_NKINDS)
+static_assert(K_NKINDS <= TUNE_KINDS, "operator kinds");
+struct GridTune {
+  int block[TUNE_KINDS];  // threads per block of each operator kind: 64, 128 or 256 (0: 256)
+  int fused[3];         // sweep of direction d: 1 reconstruction inside the flux kernel
+                        // (OpSweep), 0 face states through HBM (OpRecon + OpFlux)
+  int floor_wave;       // deferred passes: 1 one wave per face or cell, 0 one lane
+  long floor_nw;        // waves (or lanes) in flight in the deferred face passes
+  long u2p_nw;          // waves (or lanes) in flight in the deferred u2p pass
+};
+inline GridTune default_tune()
+{
+  GridTune t{};
+  for (int k = 0; k < TUNE_KINDS; k++) t.block[k] = 256;
+  for (int d = 0; d < 3; d++) t.fused[d] = 0;
+  t.floor_wave = 1;
+  t.floor_nw = 1024;
+  t.u2p_nw = 4096;
+  return t;
+}
+
 struct GridBufs {
+  GridTune tune;
   koral_params P;
   DGrid G;
   double *p, *u;
@@ -1262,8 +1380,34 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false, bool co
     dispatch(P, [&](auto t) {
       using T = decltype(t);
       constexpr int M = T::MAG, R = T::RAD;
+      const GridTune &tn = s.tune;
+      // the faces left to the floor pass (a few, each a long Newton
+      // inversion) run on a side stream while the next sweeps proceed
+      auto floor_pass = [&](int D, auto op_wave, auto op_lane) {
+        if (!M) return;
+        ex.fork(D);
+        if (tn.floor_wave)
+          ex.each_on(D, op_wave);
+        else
+          ex.each_on(D, op_lane);
+      };
+      auto fused = [&](auto dd, auto oo) {
+        constexpr int D = decltype(dd)::value, O = decltype(oo)::value;
+        ex.each(OpSweep<M, R, O, D>{P, G, s.p, s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb});
+        floor_pass(D, OpSweepFloor<M, R, O, D, true>{P, G, s.p, s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb, tn.floor_nw},
+                   OpSweepFloor<M, R, O, D, false>{P, G, s.p, s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb, tn.floor_nw});
+      };
       auto sweep = [&](auto dd) {
         constexpr int D = decltype(dd)::value;
+        if (tn.fused[D]) {
+          if (P.int_order == 4)
+            fused(dd, std::integral_constant<int, 4>{});
+          else if (P.int_order == 2)
+            fused(dd, std::integral_constant<int, 2>{});
+          else
+            fused(dd, std::integral_constant<int, 1>{});
+          return;
+        }
         if (P.int_order == 4 && D > 0)
           ex.each(OpReconRun<M, R, (D > 0 ? D : 1)>{P, G, s.p, s.fs[D], fb});
         else if (P.int_order == 4)
@@ -1273,12 +1417,8 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false, bool co
         else
           ex.each(OpRecon<M, R, 1, D>{P, G, s.p, s.fs[D], fb});
         ex.each(OpFlux<M, R, D>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb});
-        // the faces left to the floor pass (a few, each a long Newton
-        // inversion) run on a side stream while the next sweeps proceed
-        if (M) {
-          ex.fork(D);
-          ex.each_on(D, OpFluxFloor<M, R, D>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb});
-        }
+        floor_pass(D, OpFluxFloor<M, R, D, true>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb, tn.floor_nw},
+                   OpFluxFloor<M, R, D, false>{P, G, s.fs[D], s.ws, s.flb[D], s.flist[D], s.fcnt + D, fb, tn.floor_nw});
       };
       if (d == 0)
         sweep(std::integral_constant<int, 0>{});
@@ -1326,7 +1466,10 @@ void u2p(Ex &ex, const GridBufs &s, bool count_cleared = false)
     if (!count_cleared) ex.each(OpZeroU32{s.fcnt + 3, 1});
     const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy};
     ex.count(op, s.counters + 3);
-    ex.count(OpU2PFloor<T::MAG, T::RAD>{op}, s.counters + 3);
+    if (s.tune.floor_wave)
+      ex.count(OpU2PFloor<T::MAG, T::RAD, true>{op, s.tune.u2p_nw}, s.counters + 3);
+    else
+      ex.count(OpU2PFloor<T::MAG, T::RAD, false>{op, s.tune.u2p_nw}, s.counters + 3);
   });
   fixup(ex, s, F_HDFIXUP);
   if (s.P.rad) fixup(ex, s, F_RADFIXUP);

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <unistd.h>
+#include <string>
 #include <vector>
 #include "koral_hip.h"
 #include "grid_ops.hpp"
@@ -16,12 +17,15 @@
 using namespace kd;
 
 static_assert(IMP_NCTR == IMP_LIST_COUNTERS, "implicit pass counters");
+static_assert(sizeof(koral_grid_tune) == sizeof(GridTune) && sizeof(((koral_grid_tune *)0)->block) ==
+                  sizeof(((GridTune *)0)->block), "koral_grid_tune mirrors GridTune");
 
 namespace {
 
-constexpr int BS = 256;
+constexpr int BS = 256;  // largest block of the grid kernels (launch bounds)
 
 inline dim3 blocks_for(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
+inline dim3 blocks_for(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
 // Op::waves_per_simd (default 1) asks the compiler to fit that many waves
 // per SIMD in the register file
@@ -34,10 +38,12 @@ struct WavesPerSimd<Op, decltype((void)Op::waves_per_simd, void())> {
   static constexpr int value = Op::waves_per_simd;
 };
 
+// the grid kernels run with 64, 128 or 256 threads per block (the tuned
+// launch choice of each operator kind)
 template <class Op>
 __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_each(Op op)
 {
-  long i = (long)blockIdx.x * BS + threadIdx.x;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < op.size()) op(i);
 }
 
@@ -52,7 +58,7 @@ template <class Op>
 __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_max(Op op, double *out)
 {
   __shared__ double red[BS / 64];
-  long i = (long)blockIdx.x * BS + threadIdx.x;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.;
   if (i < op.size()) v = op(i);
   for (int off = 32; off > 0; off >>= 1) v = mx(v, __shfl_xor(v, off));
@@ -60,7 +66,8 @@ __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_max(Op op, doub
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = red[0];
-    for (int k = 1; k < BS / 64; k++) m = mx(m, red[k]);
+    const int nwav = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nwav; k++) m = mx(m, red[k]);
     if (m > 0.) atomic_max_pos(out, m);
   }
 }
@@ -69,7 +76,7 @@ __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_max(Op op, doub
 template <class Op>
 __global__ void __launch_bounds__(BS, WavesPerSimd<Op>::value) k_count(Op op, int *out)
 {
-  long i = (long)blockIdx.x * BS + threadIdx.x;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   Cnt3 c{{0, 0, 0}};
   if (i < op.size()) c = op(i);
 #pragma unroll
@@ -160,7 +167,14 @@ struct GridCells {
 struct HipExec {
   hipStream_t s;
   Timing *tm;
+  const GridTune *tune = nullptr;  // threads per block of each operator kind
   int rc = 0;  // first launch-configuration error
+  template <class Op>
+  int bs() const
+  {
+    const int b = tune ? tune->block[Op::kind] : BS;
+    return (b == 64 || b == 128 || b == 256) ? b : BS;
+  }
   // side streams and their fork/join events (work that may overlap the
   // main stream, e.g. the few faces of a sweep left to the floor pass)
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
@@ -207,22 +221,26 @@ struct HipExec {
   void each_on(int k, const Op &op)
   {
     hipStream_t st = side[k];
-    timed(op, [&](long n) { k_each<Op><<<blocks_for(n), BS, 0, st>>>(op); }, st);
+    const int b = bs<Op>();
+    timed(op, [&](long n) { k_each<Op><<<blocks_for(n, b), b, 0, st>>>(op); }, st);
   }
   template <class Op>
   void each(const Op &op)
   {
-    timed(op, [&](long n) { k_each<Op><<<blocks_for(n), BS, 0, s>>>(op); });
+    const int b = bs<Op>();
+    timed(op, [&](long n) { k_each<Op><<<blocks_for(n, b), b, 0, s>>>(op); });
   }
   template <class Op>
   void max(const Op &op, double *out)
   {
-    timed(op, [&](long n) { k_max<Op><<<blocks_for(n), BS, 0, s>>>(op, out); });
+    const int b = bs<Op>();
+    timed(op, [&](long n) { k_max<Op><<<blocks_for(n, b), b, 0, s>>>(op, out); });
   }
   template <class Op>
   void count(const Op &op, int *out)
   {
-    timed(op, [&](long n) { k_count<Op><<<blocks_for(n), BS, 0, s>>>(op, out); });
+    const int b = bs<Op>();
+    timed(op, [&](long n) { k_count<Op><<<blocks_for(n, b), b, 0, s>>>(op, out); });
   }
   // the implicit sweep as six per-attempt persistent passes
   ImpBufs imp{};
@@ -332,6 +350,7 @@ struct koral_hip_grid {
   HipExec ex()
   {
     HipExec e{s, &timing};
+    e.tune = &b.tune;
     for (int k = 0; k < 3; k++) {
       e.side[k] = side[k];
       e.ev_fork[k] = ev_fork[k];
@@ -423,6 +442,7 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
     return set_error(KORAL_E_ARG, "grid_create: missing metric tables");
   auto *g = new koral_hip_grid();
   GridBufs &s = g->b;
+  s.tune = default_tune();
   s.P = *P;
   DGrid &G = s.G;
   grid_layout(D, G);
@@ -591,7 +611,7 @@ const char *koral_hip_grid_op_name(int kind)
 {
   static const char *names[K_NKINDS] = {"p2u", "bc_faces", "bc_edge", "bc_total", "wavespeeds", "flux",
                                         "emf", "ct_faces", "update", "u2p", "fixup", "fixup_apply",
-                                        "implicit", "comb", "polar", "recon", "flux_floor"};
+                                        "implicit", "comb", "polar", "recon", "flux_floor", "u2p_floor", "zero"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "";
 }
 
@@ -829,6 +849,189 @@ int koral_hip_run_rk2imex(koral_hip_grid *g, int nsteps, double *t, double *dt, 
   if (!rc) {
     *t = clk[0];
     *dt = clk[1];
+  }
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+void tune_to_c(const GridTune &a, koral_grid_tune *t)
+{
+  for (int k = 0; k < TUNE_KINDS; k++) t->block[k] = a.block[k];
+  for (int d = 0; d < 3; d++) t->fused[d] = a.fused[d];
+  t->floor_wave = a.floor_wave;
+  t->floor_nw = a.floor_nw;
+  t->u2p_nw = a.u2p_nw;
+}
+
+int tune_from_c(const koral_grid_tune *t, GridTune &a)
+{
+  for (int k = 0; k < TUNE_KINDS; k++)
+    if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
+      return set_error(KORAL_E_ARG, "set_tune: block sizes must be 64, 128 or 256");
+  if (t->floor_nw < 1 || t->floor_nw > (1L << 20) || t->u2p_nw < 1 || t->u2p_nw > (1L << 20))
+    return set_error(KORAL_E_ARG, "set_tune: waves in flight must be in [1, 2^20]");
+  for (int k = 0; k < TUNE_KINDS; k++) a.block[k] = t->block[k];
+  for (int d = 0; d < 3; d++) a.fused[d] = t->fused[d] ? 1 : 0;
+  a.floor_wave = t->floor_wave ? 1 : 0;
+  a.floor_nw = t->floor_nw;
+  a.u2p_nw = t->u2p_nw;
+  return 0;
+}
+
+// state the autotuner's trial steps change, saved once and restored before
+// every trial
+struct TuneSnap {
+  double *p = nullptr, *u = nullptr, *clock = nullptr;
+  signed char *fl = nullptr;
+  ~TuneSnap()
+  {
+    if (p) (void)hipFree(p);
+    if (u) (void)hipFree(u);
+    if (clock) (void)hipFree(clock);
+    if (fl) (void)hipFree(fl);
+  }
+  int save(const GridBufs &b, hipStream_t s)
+  {
+    const size_t nvc = (size_t)b.P.nv * b.G.ncell;
+    int rc = check_hip(hipMalloc(&p, nvc * sizeof(double)), "autotune snapshot");
+    if (!rc) rc = check_hip(hipMalloc(&u, nvc * sizeof(double)), "autotune snapshot");
+    if (!rc) rc = check_hip(hipMalloc(&clock, 4 * sizeof(double)), "autotune snapshot");
+    if (!rc) rc = check_hip(hipMalloc(&fl, (size_t)NFLAGS * b.G.ncell), "autotune snapshot");
+    if (!rc) rc = copy(b.p, b.u, b.clock, b.fl, p, u, clock, fl, b, s);
+    return rc;
+  }
+  int restore(const GridBufs &b, hipStream_t s) { return copy(p, u, clock, fl, b.p, b.u, b.clock, b.fl, b, s); }
+  static int copy(const double *p0, const double *u0, const double *c0, const signed char *f0, double *p1,
+                  double *u1, double *c1, signed char *f1, const GridBufs &b, hipStream_t s)
+  {
+    const size_t nvc = (size_t)b.P.nv * b.G.ncell;
+    int rc = check_hip(hipMemcpyAsync(p1, p0, nvc * sizeof(double), hipMemcpyDeviceToDevice, s), "snapshot copy");
+    if (!rc) rc = check_hip(hipMemcpyAsync(u1, u0, nvc * sizeof(double), hipMemcpyDeviceToDevice, s), "snapshot copy");
+    if (!rc) rc = check_hip(hipMemcpyAsync(c1, c0, 4 * sizeof(double), hipMemcpyDeviceToDevice, s), "snapshot copy");
+    if (!rc)
+      rc = check_hip(hipMemcpyAsync(f1, f0, (size_t)NFLAGS * b.G.ncell, hipMemcpyDeviceToDevice, s), "snapshot copy");
+    return rc;
+  }
+};
+
+// ms per step of 1 + steps steps from the snapshot, the last `steps` timed
+int tune_trial(koral_hip_grid *g, TuneSnap &snap, int steps, hipEvent_t e0, hipEvent_t e1, float *ms)
+{
+  int rc = snap.restore(g->b, g->s);
+  HipExec ex = g->ex();
+  for (int k = 0; k < 1 + steps && !rc; k++) {
+    if (k == 1) rc = check_hip(hipEventRecord(e0, g->s), "event");
+    k_zero_scalars<<<1, 64, 0, g->s>>>(g->b.tsd, g->b.counters, 8);
+    step_rk2imex(ex, g->b);
+    k_clock_advance<<<1, 1, 0, g->s>>>(g->b.clock, g->b.tsd);
+    rc = ex.rc ? ex.rc : check_launch("autotune step");
+  }
+  if (!rc) rc = check_hip(hipEventRecord(e1, g->s), "event");
+  if (!rc) rc = check_hip(hipEventSynchronize(e1), "event sync");
+  if (!rc) rc = check_hip(hipEventElapsedTime(ms, e0, e1), "event time");
+  if (!rc) *ms /= (float)(steps > 0 ? steps : 1);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t)
+{
+  if (!g || !t) return set_error(KORAL_E_ARG, "get_tune: bad arguments");
+  tune_to_c(g->b.tune, t);
+  return 0;
+}
+
+int koral_hip_grid_set_tune(koral_hip_grid *g, const koral_grid_tune *t)
+{
+  if (!g || !t) return set_error(KORAL_E_ARG, "set_tune: bad arguments");
+  GridTune a = g->b.tune;
+  int rc = tune_from_c(t, a);
+  if (rc) return rc;
+  g->b.tune = a;
+  g->drop_graph();  // a captured step holds the old launch shapes
+  return 0;
+}
+
+int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, char *report, int report_len)
+{
+  if (!g || steps < 1 || steps > 100 || !(dt > 0.) || (report && report_len < 1))
+    return set_error(KORAL_E_ARG, "autotune: bad arguments");
+  int rc = refuse_mag_rad(g, "autotune");
+  if (!rc) rc = set_clock(g, t, dt);
+  if (rc) return rc;
+  std::string rep;
+  TuneSnap snap;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = snap.save(g->b, g->s);
+  if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+  const bool timing_was = g->timing.on;
+  g->timing.on = false;
+  g->drop_graph();
+  GridTune &T = g->b.tune;
+  float best = 0.f;
+  char line[160];
+  if (!rc) rc = tune_trial(g, snap, steps, e0, e1, &best);
+  if (!rc) {
+    snprintf(line, sizeof line, "start %.4f\n", best);
+    rep += line;
+  }
+  // one coordinate: try each value, keep the fastest if it beats the current
+  // one by at least 1 %
+  auto coord = [&](const char *name, auto get, auto set, std::initializer_list<long> vals) {
+    if (rc) return;
+    const long cur = get();
+    long pick = cur;
+    float pick_ms = best;
+    for (long v : vals) {
+      if (v == cur || rc) continue;
+      set(v);
+      float ms = 0.f;
+      rc = tune_trial(g, snap, steps, e0, e1, &ms);
+      snprintf(line, sizeof line, "%s %ld %.4f\n", name, v, ms);
+      rep += line;
+      if (!rc && ms < pick_ms * 0.99f) {
+        pick = v;
+        pick_ms = ms;
+      }
+    }
+    set(pick);
+    best = pick_ms;
+    snprintf(line, sizeof line, "chose %s %ld %.4f\n", name, pick, best);
+    rep += line;
+  };
+  const DGrid &G = g->b.G;
+  const int n3[3] = {G.nx, G.ny, G.nz};
+  const char *fname[3] = {"fused_x1", "fused_x2", "fused_x3"};
+  for (int d = 0; d < 3; d++)
+    if (n3[d] > 1)
+      coord(fname[d], [&] { return (long)T.fused[d]; }, [&](long v) { T.fused[d] = (int)v; }, {0, 1});
+  if (g->b.P.mag) {
+    coord("floor_wave", [&] { return (long)T.floor_wave; }, [&](long v) { T.floor_wave = (int)v; }, {1, 0});
+    coord("floor_nw", [&] { return T.floor_nw; }, [&](long v) { T.floor_nw = v; }, {256, 1024, 4096});
+  }
+  coord("u2p_nw", [&] { return T.u2p_nw; }, [&](long v) { T.u2p_nw = v; }, {1024, 4096, 16384});
+  // block sizes of the operator kinds a step launches
+  const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
+                       K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};
+  for (int k : kinds) {
+    snprintf(line, sizeof line, "block_%s", koral_hip_grid_op_name(k));
+    const std::string nm = line;
+    coord(nm.c_str(), [&] { return (long)T.block[k]; }, [&](long v) { T.block[k] = (int)v; }, {64, 128, 256});
+  }
+  if (!rc) rc = snap.restore(g->b, g->s);
+  if (!rc) rc = check_hip(hipStreamSynchronize(g->s), "sync");
+  g->timing.on = timing_was;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (report) {
+    snprintf(report, (size_t)report_len, "%s", rep.c_str());
   }
   return rc;
 }

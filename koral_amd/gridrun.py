@@ -16,6 +16,12 @@ _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
+class GridTune(ctypes.Structure):
+    """koral_grid_tune: the launch choices of a grid (include/koral_hip.h)."""
+    _fields_ = [("block", ctypes.c_int * 32), ("fused", ctypes.c_int * 3), ("floor_wave", ctypes.c_int),
+                ("floor_nw", ctypes.c_long), ("u2p_nw", ctypes.c_long)]
+
+
 class GridDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
                 ("bc", ctypes.c_int * 6), ("xb", _dp * 3),
@@ -49,6 +55,9 @@ def _sig(L):
         "koral_hip_grid_use_graph": [_vp, i],
         "koral_hip_grid_write_restart": [_vp, ctypes.c_char_p, i, i, d, i],
         "koral_hip_grid_read_restart": [_vp, ctypes.c_char_p, i, _vp, _vp, _vp],
+        "koral_hip_grid_get_tune": [_vp, _vp],
+        "koral_hip_grid_set_tune": [_vp, _vp],
+        "koral_hip_grid_autotune": [_vp, d, d, i, ctypes.c_char_p, i],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -214,6 +223,32 @@ class DeviceGrid:
         check(r, "read_restart")
         self.t = t.value
         return True, n1.value
+
+    def get_tune(self):
+        t = GridTune()
+        check(self.L.koral_hip_grid_get_tune(self.h, ctypes.byref(t)), "get_tune")
+        return t
+
+    def set_tune(self, t):
+        check(self.L.koral_hip_grid_set_tune(self.h, ctypes.byref(t)), "set_tune")
+
+    def autotune(self, steps=3):
+        """Searches the launch choices on the current state at (self.t,
+        self.dt) and keeps the fastest; the state is left unchanged.  Returns
+        the search log as a list of (name, value, ms per step)."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(self.L.koral_hip_grid_autotune(self.h, float(self.t), float(self.dt), int(steps), buf, len(buf)),
+              "autotune")
+        rows = []
+        for line in buf.value.decode().splitlines():
+            parts = line.split()
+            if len(parts) == 2:
+                rows.append((parts[0], None, float(parts[1])))
+            elif len(parts) == 3:
+                rows.append((parts[0], int(parts[1]), float(parts[2])))
+            elif len(parts) == 4:
+                rows.append((parts[0] + " " + parts[1], int(parts[2]), float(parts[3])))
+        return rows
 
     def use_graph(self, enable=True):
         """Replay one captured step per step in run() (needs a stream)."""

@@ -806,6 +806,8 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
   o_set_bc(P, G, u, p);
   long ncell = (long)G->sx * G->sy * G->sz;
   double *ws = o_work(W_WS, ncell * NWS);
+  /* the intermediates handed out are zero where the operator does not write */
+  if (ws_out) memset(ws, 0, sizeof(double) * ncell * NWS);
   double tmax = calc_wavespeeds(P, G, p, ws, NULL);
   if (tstepdenmax && tmax > *tstepdenmax) *tstepdenmax = tmax;
   double *flb[3] = {NULL, NULL, NULL};
@@ -815,6 +817,7 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
   for (int d = 0; d < 3; d++) {
     if (n[d] == 1) continue;
     flb[d] = o_work(W_FLBX + d, nf[d] * nv);
+    if (flb_out) memset(flb[d], 0, sizeof(double) * nf[d] * nv);
     sweep(P, G, p, ws, d, flb[d]);
   }
   if (P->mag) flux_ct(P, G, flb[0], flb[1], flb[2]);

@@ -449,6 +449,15 @@ int main(int argc, char **argv)
   D.ng = hdr[4];
   for (int k = 0; k < 6; k++) D.bc[k] = hdr[5 + k];
   GridBufs s{};
+  s.tune = default_tune();
+  // KORAL_HC_TUNE selects launch choices that change which functors run:
+  // "fused" (reconstruction inside the flux functor for every direction),
+  // "lane" (one lane per face or cell in the deferred passes), or both
+  if (const char *tn = getenv("KORAL_HC_TUNE")) {
+    if (strstr(tn, "fused"))
+      for (int d = 0; d < 3; d++) s.tune.fused[d] = 1;
+    if (strstr(tn, "lane")) s.tune.floor_wave = 0;
+  }
   s.P = P;
   DGrid &G = s.G;
   grid_layout(&D, G);

@@ -54,8 +54,12 @@ def read_output(path, spec, nv, nrec):
     return p, u, fl, recs
 
 
-def run(exe, inp, out, mode, stride, cmd, arg=None, timeout=1800, threads=None):
+def run(exe, inp, out, mode, stride, cmd, arg=None, timeout=1800, threads=None, tune=None):
+    """tune: launch choices that change which functors run ("fused",
+    "lane" or both, comma separated)."""
     env = dict(os.environ)
+    if tune:
+        env["KORAL_HC_TUNE"] = tune
     env.setdefault("ASAN_OPTIONS", "detect_leaks=0:abort_on_error=1")
     env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
     if threads:

@@ -331,3 +331,75 @@ def test_grid_restart_files(oracle, tmp_path):
     assert np.array_equal(pb[sl].transpose(2, 1, 0, 3).reshape(-1, P.nv), pnew)
     # no file: the reference starts from scratch
     assert b.read_restart(tmp_path / "none", 9) == (False, 0)
+
+
+def _steps(dg, n):
+    tot = np.zeros(6, np.int64)
+    for _ in range(n):
+        dg.step()
+        tot += dg.counters
+    pd, ud, fd = dg.download()
+    return pd, ud, fd, tot, dg.dt
+
+
+@pytest.mark.parametrize("kind", ["tube", "torus_mhd_bfloor"])
+def test_grid_launch_choices_bit_identical(kind):
+    """Every launch choice of the autotuner's space computes the same state:
+    reconstruction inside the flux kernel, one lane instead of one wave per
+    deferred face or cell, other block sizes and waves in flight; two steps
+    of each, bit for bit the state, flags, counters and next dt of the
+    default choices."""
+    from koral_amd.gridrun import DeviceGrid
+    P, spec, p = _setup(kind)
+    ref = DeviceGrid(P, spec)
+    ref.upload(p)
+    ref.init()
+    want = _steps(ref, 2)
+
+    def variant(fused, wave, block, floor_nw, u2p_nw):
+        dg = DeviceGrid(P, spec)
+        t = dg.get_tune()
+        for d in range(3):
+            t.fused[d] = fused
+        t.floor_wave = wave
+        for k in range(32):
+            t.block[k] = block
+        t.floor_nw, t.u2p_nw = floor_nw, u2p_nw
+        dg.set_tune(t)
+        dg.upload(p)
+        dg.init()
+        return _steps(dg, 2)
+
+    for args in [(1, 1, 256, 1024, 4096), (1, 0, 64, 256, 1024), (0, 0, 128, 4096, 16384), (1, 1, 128, 64, 64)]:
+        _say("launch choices fused=%d wave=%d block=%d floor_nw=%d u2p_nw=%d" % args)
+        got = variant(*args)
+        for a, b in zip(got[:3], want[:3]):
+            assert np.array_equal(a, b, equal_nan=True)
+        assert np.array_equal(got[3], want[3]) and got[4] == want[4]
+
+
+def test_grid_autotune_keeps_state_and_results():
+    """The autotuner searches on a snapshot of the state: the state after it
+    is the state before it, and steps with the chosen launch choices equal
+    steps with the defaults bit for bit."""
+    from koral_amd.gridrun import DeviceGrid
+    P, spec, p = _setup("torus_mhd_bfloor")
+    a, b = DeviceGrid(P, spec), DeviceGrid(P, spec)
+    for dg in (a, b):
+        dg.upload(p)
+        dg.init()
+    before = a.download()
+    rows = a.autotune(steps=2)
+    after = a.download()
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y, equal_nan=True)
+    chosen = [r for r in rows if r[0].startswith("chose")]
+    _say("autotune: %d trials, %d coordinates; start %.4f ms, end %.4f ms" %
+         (len(rows) - len(chosen) - 1, len(chosen), rows[0][2], chosen[-1][2]))
+    assert rows[0][0] == "start" and len(chosen) >= 5
+    t = a.get_tune()
+    _say("  chose fused %s floor_wave %d floor_nw %d u2p_nw %d" % (list(t.fused), t.floor_wave, t.floor_nw, t.u2p_nw))
+    ga, gb = _steps(a, 2), _steps(b, 2)
+    for x, y in zip(ga[:3], gb[:3]):
+        assert np.array_equal(x, y, equal_nan=True)
+    assert np.array_equal(ga[3], gb[3]) and ga[4] == gb[4]

@@ -172,3 +172,29 @@ def test_host_full_size_grrmhd_m1(tmp_path):
     for mode in (0, 1):
         r = H.run(exe, inp, tmp_path / "out.bin", mode, 1, "step", 1, timeout=7200)
         assert r.returncode == 0, r.stderr[-4000:]
+
+
+@pytest.mark.parametrize("kind", ["tube", "torus_mhd", "torus_rad"])
+@pytest.mark.parametrize("tune", ["fused", "fused,lane", "lane"])
+def test_host_step_launch_choices_match_oracle(oracle, exe, tmp_path, kind, tune):
+    """The launch choices the autotuner picks between run different functors
+    (reconstruction inside the flux functor, one lane instead of one wave per
+    deferred face or cell); each gives the oracle's state bit for bit."""
+    P, spec, p = _setup(kind)
+    nsteps = 2
+    S = oracle.GridState(P, oracle.Grid(spec), p)
+    S.init_u()
+    S.set_bc()
+    dt = 1. / S.tstepden()
+    t = 0.
+    for _ in range(nsteps):
+        tm, c = S.step(t, dt)
+        t += dt
+        dt = 1. / tm
+    inp = tmp_path / "in.bin"
+    out = tmp_path / "out.bin"
+    H.write_input(inp, P, spec, p)
+    r = H.run(exe, inp, out, 0, 1, "step", nsteps, timeout=900, tune=tune)
+    assert r.returncode == 0, r.stderr[-4000:]
+    pd, ud, fd, recs = H.read_output(out, spec, P.nv, nsteps)
+    assert _same(pd, S.p) and _same(ud, S.u) and _same(fd, S.flags), (kind, tune)
