@@ -24,6 +24,21 @@ def build(out, sanitize=True):
     return str(out)
 
 
+CLANG = "/opt/rocm/lib/llvm/bin/clang++"
+
+
+def build_msan(out):
+    """clang build of the checker with MemorySanitizer (uses of uninitialised
+    values, e.g. an index or a branch computed from a variable no path set:
+    the kind of error that is harmless on a host stack and not on a GPU),
+    serial (libgomp is not instrumented)."""
+    cmd = [CLANG, "-fsanitize=memory", "-fsanitize-memory-track-origins=2", "-fno-omit-frame-pointer", "-O1", "-g",
+           "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "koral_amd", "csrc"),
+           "-I", os.path.join(ROOT, "include"), SRC, "-o", str(out)]
+    subprocess.run(cmd, check=True)
+    return str(out)
+
+
 def write_input(path, P, spec, p_aos, u_aos=None):
     import ctypes
     hdr = [MAGIC, spec.nx, spec.ny, spec.nz, spec.ng] + list(spec.bc) + [ctypes.sizeof(P)]
@@ -62,6 +77,7 @@ def run(exe, inp, out, mode, stride, cmd, arg=None, timeout=1800, threads=None, 
         env["KORAL_HC_TUNE"] = tune
     env.setdefault("ASAN_OPTIONS", "detect_leaks=0:abort_on_error=1")
     env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
+    env["MSAN_OPTIONS"] = "halt_on_error=1"
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
     argv = [exe, str(inp), str(out), str(mode), str(stride), cmd] + ([] if arg is None else [str(arg)])

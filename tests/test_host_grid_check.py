@@ -198,3 +198,30 @@ def test_host_step_launch_choices_match_oracle(oracle, exe, tmp_path, kind, tune
     assert r.returncode == 0, r.stderr[-4000:]
     pd, ud, fd, recs = H.read_output(out, spec, P.nv, nsteps)
     assert _same(pd, S.p) and _same(ud, S.u) and _same(fd, S.flags), (kind, tune)
+
+
+@pytest.mark.skipif(not os.path.exists(H.CLANG), reason="clang with MemorySanitizer not installed")
+@pytest.mark.parametrize("kind", ["torus_rad", "torus_mhd"])
+def test_host_msan_explicit_and_step(oracle, tmp_path, kind):
+    """The field + radiation explicit operator (and the GRMHD one) under
+    MemorySanitizer: no value that depends on uninitialised memory decides a
+    branch, an address or an output, in the current functors or in the
+    earlier build's launch geometry; results still the oracle's."""
+    exe = H.build_msan(tmp_path / "grid_check_msan")
+    P, spec, p = _setup(kind)
+    S = oracle.GridState(P, oracle.Grid(spec), p)
+    S.init_u()
+    S.set_bc()
+    dt = 1. / S.tstepden()
+    inp = tmp_path / "in.bin"
+    H.write_input(inp, P, spec, S.p, S.u)
+    for cmd in ("explicit", "legacy_explicit"):
+        out = tmp_path / (cmd + ".bin")
+        r = H.run(exe, inp, out, 0, 1, cmd, repr(dt), timeout=900)
+        assert r.returncode == 0, r.stderr[-4000:]
+    S.explicit(0., dt)
+    pd, ud, fd, _ = H.read_output(tmp_path / "explicit.bin", spec, P.nv, 1)
+    assert _same(ud, S.u)
+    H.write_input(inp, P, spec, p)
+    r = H.run(exe, inp, tmp_path / "step.bin", 0, 1, "step", 1, timeout=900)
+    assert r.returncode == 0, r.stderr[-4000:]
