@@ -170,7 +170,7 @@ def bench_learned(args):
     P0 = Pp.clone()
     _, info = cell.implicit(P, U, Pp, g, G, dtd)
     F, Tg, Tr = LN.training_pairs(P, P0, Pp, dtd, info)
-    del U, Pp, P0, g, G, dtd, info
+    del U, Pp, info
     torch.cuda.empty_cache()
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
@@ -185,7 +185,47 @@ def bench_learned(args):
     say("trained both networks (%s) in %.1f s, final loss gas %.4f rad %.4f" %
         (sched, t_train, hist_g[-1], hist_r[-1]))
     del F, Tg, Tr
+    # the gate: trained on whether the learned tries solve each training cell
+    t0 = time.perf_counter()
+    U, Pp = cell.p2u(P, P0, g, G), P0.clone()
+    _, info_l = LN.implicit_learned(P, gas, rad, U, Pp, g, G, dtd, maxiter=args.learned_maxiter)
+    Fg, Tgt = LN.gate_pairs(P, P0, dtd, info_l)
+    gate_pos = float((Tgt[0] > 0).float().mean())
+    # a slice of the training cells, kept to choose the gate's threshold
+    nt = min(1 << 20, P0.shape[1])
+    tune_set = (P0[:, :nt].contiguous(), g[:, :nt].contiguous(), G[:, :nt].contiguous(), dtd[:nt].contiguous())
+    del U, Pp, P0, g, G, dtd, info_l
     torch.cuda.empty_cache()
+    gate, hist_gate = LN.train(P, Fg, Tgt, LN.GATE, schedule=sched)
+    torch.cuda.synchronize()
+    del Fg, Tgt
+    torch.cuda.empty_cache()
+    # the threshold: the fastest gated solve of the training slice over a
+    # few candidates (the evaluation batch is not looked at)
+    thr_log = []
+    if args.gate_thr is None:
+        P0t, gt, Gt, dtt = tune_set
+        U0t = cell.p2u(P, P0t, gt, Gt)
+        Ut, Pt = torch.empty_like(U0t), torch.empty_like(P0t)
+
+        def reset_t():
+            Ut.copy_(U0t)
+            Pt.copy_(P0t)
+
+        for thr in (-0.4, -0.2, 0.0, 0.2, 0.4, 0.6):
+            w, _, _ = _timed_solves(lambda: LN.implicit_learned_gated(P, gas, rad, gate, Ut, Pt, gt, Gt, dtt,
+                                                                      maxiter=args.learned_maxiter, thr=thr),
+                                    reset_t, 2, 1)
+            thr_log.append((thr, w * 1e3))
+        gate_thr = min(thr_log, key=lambda x: x[1])[0]
+        del U0t, Ut, Pt
+    else:
+        gate_thr = args.gate_thr
+    del tune_set
+    torch.cuda.empty_cache()
+    t_gate = time.perf_counter() - t0
+    say("trained the gate (%.3f of the training cells solved by the learned tries) in %.1f s, final loss %.4f; "
+        "threshold %.2f from %s" % (gate_pos, t_gate, hist_gate[-1], gate_thr, thr_log))
     # ---- fresh evaluation batch ----
     n = args.cells
     pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=args.seed)
@@ -205,6 +245,13 @@ def bench_learned(args):
                                                                           maxiter=args.learned_maxiter), reset,
                                               args.steps, args.warmup)
     pb = Pp.clone()
+    say("learned solve %.2f ms" % (w_learn * 1e3))
+    w_gated, k_gated, (_, ic) = _timed_solves(lambda: LN.implicit_learned_gated(P, gas, rad, gate, U, Pp, g, G, dtd,
+                                                                                maxiter=args.learned_maxiter,
+                                                                                thr=gate_thr), reset,
+                                              args.steps, args.warmup)
+    pc = Pp.clone()
+    say("gated learned solve %.2f ms" % (w_gated * 1e3))
     # network stages alone (features, BF16 inference, guess) on the same batch
     reset()
     stream = torch.cuda.current_stream()
@@ -218,9 +265,14 @@ def bench_learned(args):
     ev[3].record(stream)
     ev[3].synchronize()
     t_feat, t_inf, t_guess = (ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3))
-    ia, ib = ia.cpu().numpy(), ib.cpu().numpy()
-    ok_a, ok_b = ia[1] >= 0, ib[1] >= 0
-    pa, pb = pa.cpu().numpy(), pb.cpu().numpy()
+    ia, ib, ic = ia.cpu().numpy(), ib.cpu().numpy(), ic.cpu().numpy()
+    ok_a, ok_b, ok_c = ia[1] >= 0, ib[1] >= 0, ic[1] >= 0
+    pa, pb, pc = pa.cpu().numpy(), pb.cpu().numpy(), pc.cpu().numpy()
+    # cells the gate sends straight to the chain: the plain solve's result
+    # bit for bit (same attempt, same iterations, same primitives)
+    gated_out = (ic[1] != 6) & (ic[5] == ia[5])
+    same_as_plain = bool(np.array_equal(pc[:, gated_out], pa[:, gated_out], equal_nan=True))
+    it_c = float(ic[5].mean())
     both = (ib[1] == 6) & (ia[1] == 0) & (ia[2] == 2) & (ib[2] == 2)
     du = np.abs(pa[1] - pb[1]) / np.abs(pa[1])
     it_a, it_b = float(ia[5].mean()), float(ib[5].mean())
@@ -228,23 +280,30 @@ def bench_learned(args):
     # per Newton iteration to the learned solve's iterations, plus the network
     flops_it = IMPLICIT_FLOPS_PER_SOLVE / it_a
     mlp_flops = 2.0 * (16 * 128 + 2 * 128 * 128 + 128 * 16)   # padded MFMA tiles, per network
-    learned_flops = flops_it * it_b + 2 * mlp_flops
-    achieved = learned_flops * n / k_learn / 1e12
+    learned_flops = flops_it * it_c + 3 * mlp_flops
+    achieved = learned_flops * n / k_gated / 1e12
     inf_tf = mlp_flops * n / t_inf / 1e12
     res = {
         "metric": METRIC,
-        "value": n / w_learn,
+        "value": n / w_gated,
         "unit": "implicit solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": w_learn * 1e3,
+        "ms_per_step": w_gated * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded)",
-        "config": {"workload": "learned-guess implicit radiation-source solve vs plain guess",
+        "config": {"workload": "learned-guess implicit radiation-source solve (gated) vs plain guess",
+                   "gated": {"solves_per_s": n / w_gated, "speedup": w_plain / w_gated,
+                             "mean_total_iters": it_c, "iteration_ratio": it_a / it_c,
+                             "solved_frac": float(ok_c.mean()), "learned_attempt_frac": float((ic[1] == 6).mean()),
+                             "gate_thr": gate_thr, "gate_thr_search_ms": thr_log,
+                             "gate_train_positive_frac": gate_pos,
+                             "gate_train_s": t_gate, "train_loss_gate": hist_gate,
+                             "chain_cells_equal_plain": same_as_plain},
                    "cells": n, "train_cells": args.train_cells, "train_pairs": int(n_pairs),
                    "schedule_lr_epochs": sched, "batch": LN.BATCH, "networks": "gas + radiation",
                    "learned_maxiter": args.learned_maxiter or P.radimpmaxiter,
@@ -263,9 +322,9 @@ def bench_learned(args):
                                   "guess": t_guess * 1e3}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "learned implicit solve: 2 x (features + BF16 inference + guess) + classify + "
-                               "16 k_imp_pass launches (one batch)",
-                     "kernel_ms": k_learn * 1e3,
+                     "kernel": "gated learned implicit solve: 3 x (features + BF16 inference) + guesses + "
+                               "gate + classify + 16 k_imp_pass launches (one batch)",
+                     "kernel_ms": k_gated * 1e3,
                      "algorithmic_flops_per_launch": learned_flops * n,
                      "flops_basis": "counted FP64 operations per plain solve / plain mean iterations x "
                                     "learned mean iterations + network MACs x 2",
@@ -478,6 +537,8 @@ def main():
     ap.add_argument("--train-cells", type=int, default=1 << 23)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--learned-maxiter", type=int, default=16)
+    ap.add_argument("--gate-thr", type=float, default=None,
+                    help="gate threshold (default: chosen on a slice of the training cells)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -132,7 +132,9 @@ int koral_hip_implicit_batch_noalloc(const koral_params *P, long n, double *uu, 
    A small network (11 -> 128 -> 128 -> 128 -> 4, GELU) maps the local state
    of a cell to corrections of the Newton start.  kind 0 (gas) corrects
    (u, u~^1..3), kind 1 (radiation) corrects (E, F~^1..3), kind 2 predicts the
-   W start of u2p (below).  Parameters are
+   W start of u2p (below), kind 3 (gate) scores in its first output whether
+   the learned tries will solve a cell (+1 yes, -1 no, least squares on the
+   gas features).  Parameters are
    float32 [koral_hip_mlp_nparam()] in the order W1[128][11], b1[128],
    W2[128][128], b2, W3[128][128], b3, W4[4][128], b4 (W[out][in]); norm is
    float32 [30]: feature mean[11], std[11], target mean[4], std[4].
@@ -179,6 +181,9 @@ int koral_hip_mlp_train_step(koral_hip_mlp *m, const float *X, const float *T, i
                              void *stream);
 /* BF16 matrix-core inference: standardised outputs Y [4][n] */
 int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, void *stream);
+/* Gate (kind 3) outputs Y applied to a gas start: gxg[i] (u of cell i) becomes NaN where the
+   de-normalised score is below thr, so the learned solve skips cell i's learned tries */
+int koral_hip_mlp_gate(koral_hip_mlp *m, long n, const float *Y, float thr, double *gxg, void *stream);
 /* Newton start gx [4][n] of the network's kind from the outputs (the old state where not
    finite or not positive) */
 int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, const float *Y, double *gx,
@@ -194,7 +199,8 @@ int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, cons
    from the old state.  info[1*n+i] = 6 marks a cell solved by a learned try; info[5*n+i]
    counts the iterations of all attempts. */
 /* The same first attempt from starts the caller provides: gxg [4][n] (u, u~^1..3) and gxr
-   [4][n] (E, F~^1..3, may be NULL for gas primitives only), device arrays. */
+   [4][n] (E, F~^1..3, may be NULL for gas primitives only), device arrays; a cell whose
+   gxg[0*n+i] is NaN has no learned try (the chain starts at once, as in the plain solve). */
 int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, double *pp, const double *g20,
                                   const double *G20, const double *dt, double global_time, const double *gxg,
                                   const double *gxr, int maxiter, int *flags, int *info, void *stream);
@@ -206,6 +212,14 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, 
                                      double *uu, double *pp, const double *g20, const double *G20,
                                      const double *dt, double global_time, int maxiter, int *flags, int *info,
                                      void *stream);
+/* koral_hip_implicit_batch_learned with a gate network (kind 3): cells it scores below thr skip
+   the learned tries and run the plain chain from the old state at once (their result is the
+   plain solve's, bit for bit). */
+int koral_hip_implicit_batch_learned_gated(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad,
+                                           koral_hip_mlp *gate, float thr, long n, double *uu, double *pp,
+                                           const double *g20, const double *G20, const double *dt,
+                                           double global_time, int maxiter, int *flags, int *info,
+                                           void *stream);
 
 
 /* ---- grid operators (device-resident state, one GPU holding the whole

@@ -71,6 +71,21 @@ def build_binding(verbose=False):
     return out
 
 
+def build_test_libs(verbose=False):
+    """Test libraries the GPU tests load (not part of the product): the
+    element-wise FP64 check of tests/gpu/k_arith_check.hip, built with the
+    library's own flags."""
+    src = os.path.join(ROOT, "tests", "gpu", "k_arith_check.hip")
+    out = os.path.join(ROOT, "tests", "gpu", "libarith_check.so")
+    cmd = [HIPCC] + COMMON + DEVICE + ["-shared", src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-4000:]))
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

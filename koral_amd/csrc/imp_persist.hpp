@@ -99,26 +99,33 @@ __global__ void __launch_bounds__(256) k_imp_classify(koral_params P, Src src, l
 }
 
 // learned classification: the primitive kind of every cell's first learned
-// try (see above); gxr may be null
+// try (see above), or, for a cell whose gas start gxg is NaN, of its first
+// chain attempt; gxr may be null
 template <int MAG, class Src>
 __global__ void __launch_bounds__(256) k_imp_learned_classify(koral_params P, Src src, long n, ImpBufs b,
+                                                              const double *__restrict__ gxg,
                                                               const double *__restrict__ gxr)
 {
   constexpr int NV = Lay<MAG, 1>::NV;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int wp = 0;
+  bool skip = false;  // no learned try: a NaN gas start (gated out)
   if (c < n) {
     src.init(c);
     wp = P_MHD;
-    if (gxr) {
+    skip = isnan(gxg[c]);
+    if (gxr || skip) {
       double uu00[NV], pp00[NV], dt;
       Geom g;
       src.load(c, uu00, pp00, g, dt);
       wp = imp_startwith<MAG, 1>(P, pp00, g);
     }
   }
-  append_index(wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
-  append_index(wp == P_RAD, (unsigned)c, b.list[1][1], &b.ctr[IMP_LEARN_LEN + 1]);
+  // a skipped cell joins attempt 0 of the chain as a failed learned try does
+  append_index(skip && wp == P_MHD, (unsigned)c, b.list[0][0], &b.ctr[0]);
+  append_index(skip && wp == P_RAD, (unsigned)c, b.list[0][1], &b.ctr[1]);
+  append_index(!skip && wp == P_MHD, (unsigned)c, b.list[1][0], &b.ctr[IMP_LEARN_LEN]);
+  append_index(!skip && wp == P_RAD, (unsigned)c, b.list[1][1], &b.ctr[IMP_LEARN_LEN + 1]);
 }
 
 // attempt a on the cells of one list, all iterating on primitives WP with
@@ -322,7 +329,7 @@ int implicit_passes(const koral_params &P, const Src &src, long n, const ImpBufs
   k_zero_u32<<<1, 64, 0, s>>>(b.ctr, IMP_NCTR);
   int rc = 0;
   if (gxg) {
-    k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxr);
+    k_imp_learned_classify<MAG, Src><<<nb, 256, 0, s>>>(P, src, n, b, gxg, gxr);
     koral_params Pl = P;
     if (learned_maxiter > 0 && learned_maxiter < Pl.radimpmaxiter) Pl.radimpmaxiter = learned_maxiter;
     // without a radiation guess there is no second try: b.second must then be null

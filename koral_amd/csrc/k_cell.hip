@@ -540,13 +540,15 @@ int koral_hip_implicit_batch_from(const koral_params *P, long n, double *uu, dou
   return check_launch("koral_hip_implicit_batch_from");
 }
 
-int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
-                                     double *uu, double *pp, const double *g20, const double *G20,
-                                     const double *dt, double global_time, int maxiter, int *flags, int *info,
-                                     void *stream)
+namespace {
+
+// the learned solve with an optional gate network
+int learned_solve(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, koral_hip_mlp *gate, float thr,
+                  long n, double *uu, double *pp, const double *g20, const double *G20, const double *dt,
+                  double global_time, int maxiter, int *flags, int *info, void *stream, const char *what)
 {
   if (check_params(P) || n < 0 || !gas) return set_error(KORAL_E_ARG, "implicit_batch_learned: bad arguments");
-  if (koral_hip_mlp_kind(gas) != 0 || (rad && koral_hip_mlp_kind(rad) != 1))
+  if (koral_hip_mlp_kind(gas) != 0 || (rad && koral_hip_mlp_kind(rad) != 1) || (gate && koral_hip_mlp_kind(gate) != 3))
     return set_error(KORAL_E_ARG, "implicit_batch_learned: networks of the wrong kind");
   if (!P->rad) return set_error(KORAL_E_ARG, "implicit solve needs a radiation layout");
   if (n == 0) return 0;
@@ -562,6 +564,9 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, 
   if (!rc && rad) rc = koral_hip_mlp_features(P, rad, n, pp, dt, set->feat, stream);
   if (!rc && rad) rc = koral_hip_mlp_infer(rad, set->feat, n, set->y, stream);
   if (!rc && rad) rc = koral_hip_mlp_guess(rad, n, P->nv, pp, set->y, set->gxr, stream);
+  if (!rc && gate) rc = koral_hip_mlp_features(P, gate, n, pp, dt, set->feat, stream);
+  if (!rc && gate) rc = koral_hip_mlp_infer(gate, set->feat, n, set->y, stream);
+  if (!rc && gate) rc = koral_hip_mlp_gate(gate, n, set->y, thr, set->gxg, stream);
   const double *gxr = rad ? set->gxr : nullptr;
   if (!rc) {
     if (P->mag) {
@@ -575,7 +580,29 @@ int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, 
   int rc2 = release_set(set, s);
   if (rc) return rc;
   if (rc2) return rc2;
-  return check_launch("koral_hip_implicit_batch_learned");
+  return check_launch(what);
+}
+
+}  // namespace
+
+int koral_hip_implicit_batch_learned(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad, long n,
+                                     double *uu, double *pp, const double *g20, const double *G20,
+                                     const double *dt, double global_time, int maxiter, int *flags, int *info,
+                                     void *stream)
+{
+  return learned_solve(P, gas, rad, nullptr, 0.f, n, uu, pp, g20, G20, dt, global_time, maxiter, flags, info, stream,
+                       "koral_hip_implicit_batch_learned");
+}
+
+int koral_hip_implicit_batch_learned_gated(const koral_params *P, koral_hip_mlp *gas, koral_hip_mlp *rad,
+                                           koral_hip_mlp *gate, float thr, long n, double *uu, double *pp,
+                                           const double *g20, const double *G20, const double *dt,
+                                           double global_time, int maxiter, int *flags, int *info,
+                                           void *stream)
+{
+  if (!gate) return set_error(KORAL_E_ARG, "implicit_batch_learned_gated: no gate network");
+  return learned_solve(P, gas, rad, gate, thr, n, uu, pp, g20, G20, dt, global_time, maxiter, flags, info, stream,
+                       "koral_hip_implicit_batch_learned_gated");
 }
 
 }  // extern "C"

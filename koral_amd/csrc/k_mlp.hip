@@ -439,6 +439,18 @@ __global__ void __launch_bounds__(256) k_guess(long n, int base, const double *_
   gx[3 * n + c] = ok ? g3 : x3;
 }
 
+// the gate network (kind 3) scores whether a learned try will solve a cell;
+// a cell scored below thr gets no learned try: its gas start becomes NaN,
+// which the learned solve reads as "start the chain at once"
+__global__ void __launch_bounds__(256) k_gate(long n, const float *__restrict__ y, const float *__restrict__ norm,
+                                              float thr, double *__restrict__ gxg)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const float score = y[c] * norm[2 * NIN + NOUT] + norm[2 * NIN];
+  if (!(score >= thr)) gxg[c] = __builtin_nan("");
+}
+
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
@@ -528,7 +540,7 @@ int koral_hip_mlp_nparam(void) { return NPARAM; }
 
 int koral_hip_mlp_create(int kind, koral_hip_mlp **out)
 {
-  if (!out || kind < 0 || kind > 2) return set_error(KORAL_E_ARG, "mlp_create: bad arguments");
+  if (!out || kind < 0 || kind > 3) return set_error(KORAL_E_ARG, "mlp_create: bad arguments");
   auto *m = new koral_hip_mlp();
   m->kind = kind;
   int rc = 0;
@@ -701,6 +713,14 @@ int koral_hip_mlp_infer(koral_hip_mlp *m, const float *feat, long n, float *Y, v
   if (blocks > 2048) blocks = 2048;
   k_mlp_infer<<<(unsigned)blocks, 256, 0, s>>>(m->wp, m->th, feat, n, Y);
   return check_launch("mlp_infer");
+}
+
+int koral_hip_mlp_gate(koral_hip_mlp *m, long n, const float *Y, float thr, double *gxg, void *stream)
+{
+  if (!m || m->kind != 3 || n < 0 || !Y || !gxg) return set_error(KORAL_E_ARG, "mlp_gate: bad arguments");
+  if (n == 0) return 0;
+  k_gate<<<grid1(n), 256, 0, (hipStream_t)stream>>>(n, Y, m->norm, thr, gxg);
+  return check_launch("mlp_gate");
 }
 
 int koral_hip_mlp_guess(koral_hip_mlp *m, long n, int nv, const double *pp, const float *Y, double *gx,

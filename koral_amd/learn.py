@@ -25,7 +25,7 @@ from .cell import _ptr, _stream, _chk
 
 NIN, NH, NOUT = 11, 128, 4
 BATCH = 4096
-GAS, RAD, WU2P = 0, 1, 2
+GAS, RAD, WU2P, GATE = 0, 1, 2, 3
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 
@@ -49,6 +49,9 @@ def _sig(L):
         "koral_hip_mlp_guess": [vp, l, i, vp, vp, vp, vp],
         "koral_hip_implicit_batch_learned": [vp, vp, vp, l, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp,
                                              vp],
+        "koral_hip_implicit_batch_learned_gated": [vp, vp, vp, vp, f, l, vp, vp, vp, vp, vp, ctypes.c_double, i,
+                                                   vp, vp, vp],
+        "koral_hip_mlp_gate": [vp, l, vp, f, vp, vp],
         "koral_hip_implicit_batch_from": [vp, l, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, i, vp, vp, vp],
         "koral_hip_mlp_u2p_features": [vp, vp, l, vp, vp, vp, vp, vp, vp],
         "koral_hip_mlp_u2p_targets": [vp, l, vp, vp, vp, vp, vp, vp],
@@ -81,7 +84,8 @@ def init_params(seed=0):
 
 class Net:
     """One network on the device; kind GAS corrects (u, u~^i), RAD (E, F~^i),
-    WU2P predicts the W start of the u2p inversion."""
+    WU2P predicts the W start of the u2p inversion, GATE scores whether the
+    learned tries will solve a cell."""
 
     def __init__(self, kind=GAS, seed=0, theta=None, norm=None):
         self.L = _sig(lib())
@@ -321,6 +325,35 @@ def implicit_learned(P, gas, rad, uu, pp, g20, G20, dt, global_time=0.0, maxiter
                                                  int(maxiter), _ptr(flags), _ptr(info), _stream()),
           "implicit_learned")
     return flags, info
+
+
+def implicit_learned_gated(P, gas, rad, gate, uu, pp, g20, G20, dt, global_time=0.0, maxiter=0, thr=0.0):
+    """koral_hip_implicit_batch_learned_gated: cells the gate scores below
+    thr run the plain chain at once; (flags, info)."""
+    n = pp.shape[1]
+    flags = torch.zeros((7, n), dtype=torch.int32, device=pp.device)
+    info = torch.zeros((6, n), dtype=torch.int32, device=pp.device)
+    check(gas.L.koral_hip_implicit_batch_learned_gated(ctypes.byref(P), gas.h, rad.h if rad is not None else None,
+                                                       gate.h, float(thr), n, _ptr(_chk(uu)), _ptr(_chk(pp)),
+                                                       _ptr(_chk(g20)), _ptr(_chk(G20)), _ptr(_chk(dt)),
+                                                       float(global_time), int(maxiter), _ptr(flags), _ptr(info),
+                                                       _stream()), "implicit_learned_gated")
+    return flags, info
+
+
+def gate_pairs(P, pp0, dt, info_learned):
+    """Raw features [11][m] of every cell and gate targets [4][m]: +1 in the
+    first output where the learned tries solved the cell (info[1] = 6), -1
+    where they did not (the other outputs 0)."""
+    L = _sig(lib())
+    n = pp0.shape[1]
+    F = torch.empty((NIN, n), dtype=torch.float32, device=pp0.device)
+    check(L.koral_hip_mlp_features(ctypes.byref(P), None, n, _ptr(_chk(pp0)), _ptr(_chk(dt)), _ptr(F),
+                                   _stream()), "mlp_features")
+    T = torch.zeros((NOUT, n), dtype=torch.float32, device=pp0.device)
+    T[0] = torch.where(info_learned[1] == 6, 1.0, -1.0)
+    keep = torch.isfinite(F).all(0)
+    return F[:, keep].contiguous(), T[:, keep].contiguous()
 
 
 def implicit_from(P, gxg, gxr, uu, pp, g20, G20, dt, global_time=0.0, maxiter=0):

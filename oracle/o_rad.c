@@ -578,8 +578,9 @@ int o_solve_implicit_lab(const oracle_params *P, double *uu, double *pp, const o
    part from gxg.  With gxr given, a cell that attempt does not solve tries
    the other primitive set from its start the same way.  A cell a learned try solves
    reports attempt 6, otherwise the unchanged six-attempt chain runs from the
-   old state and total_iters includes the learned tries.  The learned initial
-   guess has no reference counterpart. */
+   old state and total_iters includes the learned tries.  A NaN gas start
+   (a cell the gate network leaves out) skips the learned tries.  The learned
+   initial guess has no reference counterpart. */
 int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, const o_geom *g, double dt,
                                double global_time, int *flags, o_imp_info *info, const double *gxg,
                                const double *gxr, int maxiter)
@@ -604,7 +605,7 @@ int o_solve_implicit_lab_guess(const oracle_params *P, double *uu, double *pp, c
                          {1, O_ENERGYEQ, O_FF},  {0, O_ENTROPYEQ, O_LAB}, {1, O_ENTROPYEQ, O_LAB}};
   int ret = -1, total = 0, it = 0, a = 0;
   int params[4] = {0, 0, 0, 0};
-  if (gxg) {
+  if (gxg && !isnan(gxg[0])) { /* a NaN gas start: no learned try */
     params[0] = gxr ? startwith : O_MHD;
     params[1] = O_ENERGYEQ;
     params[2] = O_LAB;

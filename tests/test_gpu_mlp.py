@@ -250,6 +250,62 @@ def test_learned_solve_matches_oracle(dev, oracle):
         assert rel.max() < 1e-9
 
 
+def test_gated_learned_solve(dev, oracle):
+    """A gate network (kind 3), trained on whether the learned tries solve a
+    training cell: the cells it scores below zero skip the learned tries and
+    give the plain solve's result bit for bit (same attempt, iterations and
+    primitives); the device matches the oracle's solve from the same starts
+    with those cells' gas start set to NaN (no learned try)."""
+    import torch
+    from koral_amd import params as KP, problems as PR, cell
+    from koral_amd import learn as LN
+    P = KP.params_implicit_batch()
+    gas, rad, _, _, _ = _trained_pair(dev, P, 1 << 16, 41, [(1e-3, 3)])
+    # gate training cells: the learned tries' outcome on a fresh batch
+    pp, g20, G20, dt = PR.implicit_batch_states(P, 1 << 16, seed=43)
+    P0, g, G, dtd = cell.to_dev(pp), cell.to_dev(g20), cell.to_dev(G20), torch.from_numpy(dt).to(dev)
+    U, Pp = cell.p2u(P, P0, g, G), P0.clone()
+    _, info_l = LN.implicit_learned(P, gas, rad, U, Pp, g, G, dtd, maxiter=16)
+    Fg, Tgt = LN.gate_pairs(P, P0, dtd, info_l)
+    gate, hist = LN.train(P, Fg, Tgt, LN.GATE, schedule=[(1e-3, 4)])
+    _say("gate trained on %d cells (%.3f positive), loss %s" % (Fg.shape[1], float((Tgt[0] > 0).float().mean()),
+                                                                 ["%.3f" % h for h in hist]))
+    # fresh evaluation batch
+    n = 3000
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=44)
+    uu = oracle.p2u(P, pp, g20, G20)
+    Pd, Ud, gd, Gd = cell.to_dev(pp), cell.to_dev(uu), cell.to_dev(g20), cell.to_dev(G20)
+    dtd = torch.from_numpy(dt).to(dev)
+    _, nm = gate.get()
+    Y = gate.infer(gate.features(P, Pd, dtd)).cpu().numpy()
+    score = Y[0] * nm[2 * LN.NIN + LN.NOUT] + nm[2 * LN.NIN]
+    skipped = ~(score >= 0.)
+    _say("gate skips %d of %d cells" % (skipped.sum(), n))
+    assert 0 < skipped.sum() < n
+    U1, P1 = Ud.clone(), Pd.clone()
+    fl_p, info_p = cell.implicit(P, U1, P1, gd, Gd, dtd)
+    U2, P2 = Ud.clone(), Pd.clone()
+    fl_g, info_g = LN.implicit_learned_gated(P, gas, rad, gate, U2, P2, gd, Gd, dtd, maxiter=16, thr=0.0)
+    info_p, info_g = info_p.cpu().numpy().T, info_g.cpu().numpy().T
+    p_p, p_g = cell.to_host(P1), cell.to_host(P2)
+    assert np.array_equal(info_g[skipped], info_p[skipped])
+    assert np.array_equal(p_g[skipped], p_p[skipped], equal_nan=True)
+    assert np.array_equal(fl_g.cpu().numpy().T[skipped], fl_p.cpu().numpy().T[skipped])
+    # the oracle from the same starts, NaN gas start where the gate skips
+    gxg = gas.start(P, Pd, dtd).cpu().numpy().T.copy()
+    gxr = rad.start(P, Pd, dtd).cpu().numpy().T.copy()
+    gxg[skipped, 0] = np.nan
+    uo_r, po_r, fl_r, info_r = oracle.implicit(P, uu, pp, g20, G20, dt, gx=gxg, maxiter=16, gxr=gxr, nthreads=8)
+    same = np.all(info_g == info_r, axis=1)
+    ok = same & (info_g[:, 1] >= 0)
+    rel = np.abs(p_g[ok] - po_r[ok]) / (np.abs(p_g[ok]) + np.abs(po_r[ok]) + 1e-300)
+    _say("gated solve vs oracle: info identical %.4f, max rel p %.2e; learned attempt %d cells" %
+         (same.mean(), rel.max(), (info_g[:, 1] == 6).sum()))
+    assert np.all(same[skipped])
+    assert same.mean() > 0.99
+    assert rel.max() < 1e-9
+
+
 # ---- the W network of the u2p inversion ----
 def _u2p_batch(dev, oracle, n, seed):
     from koral_amd import params as KP, problems as PR, cell

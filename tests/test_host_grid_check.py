@@ -225,3 +225,33 @@ def test_host_msan_explicit_and_step(oracle, tmp_path, kind):
     H.write_input(inp, P, spec, p)
     r = H.run(exe, inp, tmp_path / "step.bin", 0, 1, "step", 1, timeout=900)
     assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_host_implicit_batch_bit_identical_to_oracle(oracle, tmp_path):
+    """The device's implicit solve (dev_implicit.hpp: the per-lane Newton step
+    of the GPU passes in the sequential fallback chain) built for the host
+    gives the oracle's result bit for bit on the seeded batch workload: same
+    attempt, iterations and primitives on every cell, so a difference on the
+    GPU comes from the GPU's arithmetic, not from the algorithm."""
+    import subprocess
+    exe = tmp_path / "implicit_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-I",
+                    os.path.join(H.ROOT, "koral_amd", "csrc"), "-I", os.path.join(H.ROOT, "include"),
+                    os.path.join(H.ROOT, "tests", "host", "implicit_check.cpp"), "-o", str(exe)], check=True)
+    P = KP.params_implicit_batch()
+    n = 1 << 14
+    pp, g20, G20, dt = PR.implicit_batch_states(P, n, seed=90)
+    uu = oracle.p2u(P, pp, g20, G20)
+    inp, out = tmp_path / "batch.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(bytes(P))
+        f.write(np.int64(n).tobytes())
+        for a in (pp, uu, g20, G20, dt):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    subprocess.run([str(exe), str(inp), str(out)], check=True)
+    raw = open(out, "rb").read()
+    ph = np.frombuffer(raw[:8 * P.nv * n], np.float64).reshape(n, P.nv)
+    ih = np.frombuffer(raw[8 * P.nv * n:], np.int32).reshape(n, 6)
+    _, po, _, info = oracle.implicit(P, uu, pp, g20, G20, dt, nthreads=4)
+    assert np.array_equal(ih, info)
+    assert _same(ph, po)
