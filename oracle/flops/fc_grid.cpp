@@ -2,7 +2,9 @@
 // (the oracle's o_grid.c compiled with the counting double), on a grid set-up
 // read from a file in the host check's input format.
 #include <stdio.h>
+#include <algorithm>
 #include <initializer_list>
+#include <vector>
 #include "cd.hpp"
 #define O_FLOPCOUNT 1
 #define double cd
@@ -88,9 +90,16 @@ int fc_grid_counts(FILE *f)
   const long npl = (long)G.sy * G.sx, nplx = (long)G.sy * (G.sx + 1), nply = (long)(G.sy + 1) * G.sx;
   const cd **tabs[7] = {&G.gc, &G.Gc, &G.kr, &G.gfx, &G.Gfx, &G.gfy, &G.Gfy};
   const long tn[7] = {20 * npl, 20 * npl, 64 * npl, 20 * nplx, 20 * nplx, 20 * nply, 20 * nply};
+  // the file holds the device layout [c][point]; the oracle's tables are
+  // [point][c]
+  const long tc[7] = {20, 20, 64, 20, 20, 20, 20};
   for (int k = 0; k < 7; k++) {
+    std::vector<double> raw(tn[k]);
+    if (fread(raw.data(), sizeof(double), tn[k], f) != (size_t)tn[k]) return 2;
+    const long np = tn[k] / tc[k];
     cd *y = (cd *)malloc(sizeof(cd) * tn[k]);
-    if (fread(y, sizeof(cd), tn[k], f) != (size_t)tn[k]) return 2;
+    for (long q = 0; q < np; q++)
+      for (long c = 0; c < tc[k]; c++) y[q * tc[k] + c] = raw[c * np + q];
     *tabs[k] = y;
   }
   const int nv = P.nv;
@@ -112,6 +121,34 @@ int fc_grid_counts(FILE *f)
   const cd gamma = cd(1.) - cd(1.) / sqrt(cd(2.));
   // the operators of one step in the order of o_step_rk2imex (the stage
   // algebra is counted from its statements below)
+  if (getenv("FC_U2P_HIST")) {
+    // per-cell operation counts of one u2p call on the initial state after
+    // one explicit update (the cells the deferred pass takes stand out)
+    FcAcc b2, w2, s2[3], c2, up2;
+    fc_explicit(&P, &G, dt, u, p, b2, w2, s2, c2, up2);
+    std::vector<double> cnt;
+    const int nvv = P.nv;
+    for (int ix = 0; ix < G.nx; ix++)
+      for (int iy = 0; iy < G.ny; iy++)
+        for (int iz = 0; iz < G.nz; iz++) {
+          const long c = cidx(&G, ix, iy, iz);
+          o_geom g;
+          o_geom_cell(&G, ix, iy, &g);
+          int corr[3] = {0, 0, 0}, fix[2] = {0, 0};
+          FlopCount a = g_flops;
+          const int r = o_u2p(&P, u + (long)nvv * c, p + (long)nvv * c, &g, corr, fix);
+          cnt.push_back(fc_total(g_flops) - fc_total(a));
+          (void)r;
+        }
+    std::sort(cnt.begin(), cnt.end());
+    double tot = 0.;
+    for (double v : cnt) tot += v;
+    const size_t m = cnt.size();
+    printf("{\"cells\": %zu, \"mean\": %.1f, \"median\": %.1f, \"p99\": %.1f, \"p999\": %.1f, \"max\": %.1f, "
+           "\"top_0.1pct_share\": %.3f}\n", m, tot / m, cnt[m / 2], cnt[m * 99 / 100], cnt[m * 999 / 1000], cnt[m - 1],
+           [&] { double s = 0.; for (size_t k = m - m / 1000; k < m; k++) s += cnt[k]; return s / tot; }());
+    return 0;
+  }
   FcAcc bc, ws, sw[3], ct, upd, u2p, imp, pol;
   int counts[6] = {0};
   for (int stage = 0; stage < 2; stage++) {
