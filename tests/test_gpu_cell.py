@@ -1,6 +1,8 @@
 """GPU parity of the batched per-point kernels against the CPU oracle, through
-the C-ABI (libkoral_hip.so).  Tolerances: primitives to a relative 1e-10
-(the path's FP64 parity bar), flags and return codes identical."""
+the C-ABI (libkoral_hip.so).  The device rounds every operation as the
+oracle does (IEEE FP64 without contraction, the same portable exp/log/pow),
+so results, flags, return codes and iteration counts are identical bit for
+bit; a mismatch is reported with the cells where it happens."""
 import sys
 
 import numpy as np
@@ -30,6 +32,18 @@ def _rel(a, b):
     return np.abs(a - b) / (np.abs(a) + np.abs(b) + 1e-300)
 
 
+def _ident(what, dev, ref):
+    """dev and ref equal bit for bit (NaN where both are NaN); prints the
+    first differing points otherwise"""
+    d, r = np.asarray(dev), np.asarray(ref)
+    same = (d == r) | (np.isnan(d) & np.isnan(r)) if d.dtype.kind == "f" else (d == r)
+    if not same.all():
+        bad = np.argwhere(~same)
+        _say("  %s: %d of %d values differ, first at %s: device %r oracle %r" %
+             (what, len(bad), same.size, tuple(bad[0]), d[tuple(bad[0])], r[tuple(bad[0])]))
+    return bool(same.all())
+
+
 @pytest.mark.parametrize("mag,rad", LAYOUTS)
 def test_p2u_parity(dev, oracle, mag, rad):
     P = KP.base_params(mag, rad, 5. / 3.)
@@ -37,7 +51,7 @@ def test_p2u_parity(dev, oracle, mag, rad):
     _say("p2u parity mag=%d rad=%d n=%d" % (mag, rad, len(pp)))
     ref = oracle.p2u(P, pp, g20, G20)
     out = dev.to_host(dev.p2u(P, dev.to_dev(pp), dev.to_dev(g20), dev.to_dev(G20)))
-    assert _rel(out, ref).max() < 1e-14
+    assert _ident("p2u", out, ref)
 
 
 @pytest.mark.parametrize("mag,rad", LAYOUTS)
@@ -60,8 +74,7 @@ def test_u2p_parity(dev, oracle, mag, rad):
     po, uo = dev.to_host(Pp), dev.to_host(U)
     assert np.array_equal(ret, ret_r)
     assert np.array_equal(fl, fl_r)
-    assert _rel(po, po_r).max() < 1e-10
-    assert _rel(uo, uo_r).max() < 1e-10
+    assert _ident("u2p p", po, po_r) and _ident("u2p u", uo, uo_r)
 
 
 @pytest.mark.parametrize("etype", [0, 1])
@@ -76,7 +89,7 @@ def test_u2p_solver_parity(dev, oracle, etype):
     ret = dev.u2p_solver(P, dev.to_dev(uu), Pp, dev.to_dev(g20), dev.to_dev(G20), etype).cpu().numpy()
     assert np.array_equal(ret, ret_r)
     ok = ret == 0
-    assert _rel(dev.to_host(Pp)[ok], po_r[ok]).max() < 1e-10
+    assert _ident("u2p_solver p", dev.to_host(Pp)[ok], po_r[ok])
 
 
 def test_u2p_rad_parity(dev, oracle):
@@ -90,7 +103,7 @@ def test_u2p_rad_parity(dev, oracle):
     assert np.array_equal(ret.cpu().numpy(), ret_r)
     assert np.array_equal(corr.cpu().numpy(), corr_r)
     assert corr_r.sum() > 100  # the gamma cap path is exercised
-    assert _rel(dev.to_host(Pp), po_r).max() < 1e-10
+    assert _ident("u2p_rad p", dev.to_host(Pp), po_r)
 
 
 def test_Gi_parity(dev, oracle):
@@ -100,8 +113,7 @@ def test_Gi_parity(dev, oracle):
     for lab in (1, 0):
         ref = oracle.Gi(P, pp, g20, G20, lab)
         out = dev.to_host(dev.Gi(P, dev.to_dev(pp), dev.to_dev(g20), dev.to_dev(G20), lab))
-        scale = np.abs(ref).max(1, keepdims=True)
-        assert (np.abs(out - ref) / scale).max() < 1e-12
+        assert _ident("G^mu lab=%d" % lab, out, ref)
 
 
 def test_implicit_parity(dev, oracle):
@@ -119,13 +131,12 @@ def test_implicit_parity(dev, oracle):
     torch.cuda.synchronize()
     fl, info = fl.cpu().numpy().T, info.cpu().numpy().T
     po, uo = dev.to_host(Pp), dev.to_host(U)
-    same_attempt = info[:, 1] == info_r[:, 1]
-    _say("implicit: attempts equal %.5f, iterations equal %.5f, success %.4f" %
-         (same_attempt.mean(), (info[:, 0] == info_r[:, 0]).mean(), (info[:, 1] >= 0).mean()))
-    # cells whose path diverged (different attempt) are listed, and must be rare
-    assert same_attempt.mean() > 0.999
-    both = same_attempt & (info[:, 1] >= 0)
-    rel = _rel(po[both], po_r[both]).max(1)
-    _say("implicit: max rel on primitives %.3e, 99.9%% %.3e" % (rel.max(), np.quantile(rel, 0.999)))
-    assert np.quantile(rel, 0.999) < 1e-10
-    assert np.array_equal(fl[same_attempt][:, [1, 3, 6]], fl_r[same_attempt][:, [1, 3, 6]])
+    same = np.all(info == info_r, axis=1)
+    _say("implicit: info identical %.5f, success %.4f" % (same.mean(), (info[:, 1] >= 0).mean()))
+    for i in np.where(~same)[0][:10]:
+        _say("  cell %d: device info %s oracle %s" % (i, info[i], info_r[i]))
+    # attempt, iterations and the chosen (primitives, equation, frame) of
+    # every cell, its flags, primitives and conserved state: bit for bit
+    assert same.all()
+    assert _ident("implicit flags", fl, fl_r)
+    assert _ident("implicit p", po, po_r) and _ident("implicit u", uo, uo_r)

@@ -1,7 +1,8 @@
 """Parity at the workloads' full sizes: the 1-D radiation shock tube (512
-cells, 1000 RK2IMEX steps at Courant 0.5, PPM + HLL + implicit source) and
-one RK2IMEX step of the GRMHD torus at 128x128x64 (MP5, HLL, flux-CT,
-u2p), device against the CPU oracle from the same seeded set-up."""
+cells, 1000 RK2IMEX steps at Courant 0.5, PPM + HLL + implicit source), one
+RK2IMEX step of the GRMHD torus at 128x128x64 (MP5, HLL, flux-CT, u2p) and
+the 2^22-cell implicit batch, device against the CPU oracle from the same
+seeded set-up: identical bit for bit (state, flags, counters, time step)."""
 import sys
 
 import numpy as np
@@ -11,31 +12,20 @@ from koral_amd import params as KP, problems as PR, grid as GR
 
 pytestmark = pytest.mark.gpu
 
-FLOOR = 1e-4
-
-
 def _say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
 
 
-def _rel(dev, ora):
-    """max over variables of |a - b| / (|a| + |b| + FLOOR max|b|)"""
-    worst = 0.
-    for iv in range(dev.shape[-1]):
-        a, b = dev[..., iv], ora[..., iv]
-        r = np.abs(a - b) / (np.abs(a) + np.abs(b) + FLOOR * np.abs(b).max() + 1e-300)
-        worst = max(worst, float(r.max()))
-    return worst
+def _ndiff(dev, ora):
+    """number of values that differ bit for bit (NaN equal to NaN)"""
+    return int((~((dev == ora) | (np.isnan(dev) & np.isnan(ora)))).sum())
 
 
 def test_shock_tube_1000_steps(oracle):
-    """Every 100 steps of the 1000-step oracle run, the device takes the
-    oracle's state and both advance one step: they agree to the per-step
-    bar (flags identical).  A free-running device copy is compared with the
-    oracle at the same checkpoints; its drift (the implicit solve's last
-    digits differ, so dt differs in the 14th digit after a few steps) is
-    reported and bounded."""
+    """The device runs the 1000 steps on its own (time kept on the device,
+    dt of each step from the last) and matches the oracle's 1000 steps bit
+    for bit at every 100th step: state, flags and time."""
     from koral_amd.gridrun import DeviceGrid
     P = KP.params_tube()
     spec, p = PR.tube_setup(P, nx=512)
@@ -48,36 +38,20 @@ def test_shock_tube_1000_steps(oracle):
     free.upload(p)
     free.init()
     assert free.dt == dt
-    probe = DeviceGrid(P, spec)
     t = 0.
-    worst_step, worst_free = 0., 0.
     for block in range(10):
-        for _ in range(99):
+        for _ in range(100):
             tm, _ = S.step(t, dt)
             t += dt
             dt = 1. / tm
-        # one step from the oracle's state on both sides
-        probe.upload(S.p, S.u, S.flags)
-        probe.t, probe.dt = t, dt
-        tm, _ = S.step(t, dt)
-        probe.step()
-        t += dt
-        dt = 1. / tm
-        pd, ud, fd = probe.download()
-        rs = max(_rel(pd, S.p), _rel(ud, S.u))
-        assert np.array_equal(fd, S.flags)
         free.run(100)
-        pf, uf, _ = free.download()
-        rf = max(_rel(pf, S.p), _rel(uf, S.u))
-        l1 = max(float(np.abs(pf[..., iv] - S.p[..., iv]).sum() / np.abs(S.p[..., iv]).sum())
-                 for iv in (0, 1, P.ee))
-        worst_step, worst_free = max(worst_step, rs), max(worst_free, l1)
-        _say("  step %4d t=%.6f: one-step max rel %.3e; free run max rel %.3e, L1 rel (rho, u, E) %.3e "
-             "(t %.17g vs %.17g)" % (100 * (block + 1), t, rs, rf, l1, free.t, t))
-    assert worst_step < 1e-10
-    # the free-running copy drifts where the shock fronts sit (pointwise up
-    # to ~1e-2 after 1000 steps); the profiles agree in L1
-    assert worst_free < 1e-3
+        pf, uf, ff = free.download()
+        nd = _ndiff(pf, S.p) + _ndiff(uf, S.u)
+        _say("  step %4d t=%.17g: %d values differ, flags identical %s, device t %.17g dt %.17g oracle dt %.17g" %
+             (100 * (block + 1), t, nd, np.array_equal(ff, S.flags), free.t, free.dt, dt))
+        assert nd == 0
+        assert np.array_equal(ff, S.flags)
+        assert free.t == t and free.dt == dt
 
 
 def test_grmhd_torus_full_size_step(oracle):
@@ -92,25 +66,28 @@ def test_grmhd_torus_full_size_step(oracle):
     dg = DeviceGrid(P, spec)
     dg.upload(p)
     dg.init()
-    assert abs(dg.dt - dt) <= 1e-12 * dt
-    dg.dt = dt
-    tm, c_o = S.step(0., dt)
+    assert dg.dt == dt
+    oracle.grid_set_threads(16)  # the oracle's result does not depend on it
+    try:
+        tm, c_o = S.step(0., dt)
+    finally:
+        oracle.grid_set_threads(1)
     dg.step()
     pd, ud, fd = dg.download()
-    rp, ru = _rel(pd, S.p), _rel(ud, S.u)
-    _say("  128x128x64 step: max rel p %.3e u %.3e, next dt device %.17g oracle %.17g, counters %s / %s" %
-         (rp, ru, dg.dt, 1. / tm, list(dg.counters), list(c_o)))
-    assert rp < 1e-10 and ru < 1e-10
+    nd = _ndiff(pd, S.p) + _ndiff(ud, S.u)
+    _say("  128x128x64 step: %d values differ, next dt device %.17g oracle %.17g, counters %s / %s" %
+         (nd, dg.dt, 1. / tm, list(dg.counters), list(c_o)))
+    assert nd == 0
     assert np.array_equal(fd, S.flags)
     assert np.array_equal(np.asarray(c_o), dg.counters)
-    assert abs(dg.dt - 1. / tm) <= 1e-12 / tm
+    assert dg.dt == 1. / tm
 
 
 def test_implicit_batch_full_size(oracle):
     """The batched implicit solve of the full 2^22-cell workload (the
     benchmark's seed) on the device; a random sample of 2^15 cells solved by
-    the oracle from the same inputs agrees on attempt indices and
-    iteration counts, and on the primitives to the per-step bar."""
+    the oracle from the same inputs: attempt, iterations, flags and
+    primitives identical bit for bit."""
     import torch
     from koral_amd import cell
     P = KP.params_implicit_batch()
@@ -127,9 +104,12 @@ def test_implicit_batch_full_size(oracle):
     info = info.cpu().numpy().T[idx]
     po = cell.to_host(Pp)[idx]
     same = np.all(info == info_r, axis=1)
-    ok = same & (info[:, 1] >= 0)
-    rel = (np.abs(po[ok] - po_r[ok]) / (np.abs(po[ok]) + np.abs(po_r[ok]) + 1e-300)).max(1)
-    _say("  2^22 cells, sample %d: info identical %.5f, solved %.4f, max rel %.3e, 99.9%% %.3e" %
-         (len(idx), same.mean(), (info[:, 1] >= 0).mean(), rel.max(), np.quantile(rel, 0.999)))
-    assert same.mean() > 0.999
-    assert np.quantile(rel, 0.999) < 1e-10
+    fl = fl.cpu().numpy().T[idx]
+    nd = _ndiff(po, po_r)
+    _say("  2^22 cells, sample %d: info identical %.5f, solved %.4f, %d primitive values differ" %
+         (len(idx), same.mean(), (info[:, 1] >= 0).mean(), nd))
+    for i in np.where(~same)[0][:10]:
+        _say("  cell %d: device info %s oracle %s" % (idx[i], info[i], info_r[i]))
+    assert same.all()
+    assert np.array_equal(fl, fl_r)
+    assert nd == 0

@@ -246,8 +246,9 @@ def test_learned_solve_matches_oracle(dev, oracle):
              "max rel p %.2e" % ("gas" if r is None else "gas+rad", budget, n, (info[:, 1] == 6).sum(),
                                  ((info[:, 1] == 6) & (info[:, 2] == 1)).sum(), same.mean(), rel.max()))
         assert (info[:, 1] == 6).sum() > 0
-        assert same.mean() > 0.99
-        assert rel.max() < 1e-9
+        # the same starts give the same iterates: identical bit for bit
+        assert same.all()
+        assert np.array_equal(po, po_r, equal_nan=True)
 
 
 def test_gated_learned_solve(dev, oracle):
@@ -301,9 +302,8 @@ def test_gated_learned_solve(dev, oracle):
     rel = np.abs(p_g[ok] - po_r[ok]) / (np.abs(p_g[ok]) + np.abs(po_r[ok]) + 1e-300)
     _say("gated solve vs oracle: info identical %.4f, max rel p %.2e; learned attempt %d cells" %
          (same.mean(), rel.max(), (info_g[:, 1] == 6).sum()))
-    assert np.all(same[skipped])
-    assert same.mean() > 0.99
-    assert rel.max() < 1e-9
+    assert same.all()
+    assert np.array_equal(p_g, po_r, equal_nan=True)
 
 
 # ---- the W network of the u2p inversion ----
