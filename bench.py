@@ -30,11 +30,28 @@ METRIC = "zone-updates/s (3D GRRMHD+M1, FP64); implicit solves/s learned vs plai
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
-# HBM bytes per launch of the grid kernels on the 128x128x64 torus (flux and
-# recon averaged over the three sweep directions): FETCH_SIZE and WRITE_SIZE
-# per dispatch scaled by the same counters on a plain copy of known size
-# (profiles/hbm_traffic_grmhd_torus_128x128x64.md).
-GRID_TRAFFIC_128x128x64 = {"u2p": 315e6, "flux": 445e6, "recon": 323e6}
+# HBM bytes per launch of the grid kernels on the 128x128x64 torus, per
+# operator kind and schedule of its sweeps (keys: kind, or kind + "_fused" /
+# "_split" for the face kernels, averaged over the directions run that way):
+# FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters on a
+# plain copy of known size (profiles/hbm_traffic_grmhd_torus_128x128x64.md).
+GRID_TRAFFIC_128x128x64 = {"u2p": 315e6, "flux_split": 445e6, "recon": 323e6, "update": 564e6}
+
+# FP64 operations per zone-update of the GRMHD RK2IMEX step on the
+# 128x128x64 torus, counted by the operation-counting build of the oracle
+# (oracle/flops, each add, mul, div, sqrt and transcendental one operation;
+# tests/test_oracle_flops.py re-counts a smaller grid): the three face sweeps
+# (reconstruction, floors, fluxes, HLL) of both explicit stages, u2p with
+# its fixups and ghost fill (four calls), update with metric sources, wave
+# speeds and the rest of the step.
+GRMHD_FLOPS_PER_ZONE = {"sweeps": 16581.7, "u2p": 5243.4, "update": 1356.0, "wavespeeds": 1240.6,
+                        "other": 392.0, "step": 24813.8}
+# Algorithmic HBM bytes per zone-update of the GRMHD step in the fused
+# schedule of the scope table (K1: explicit1 + u2p, K2: explicit2 + u2p; 9
+# NV-vectors of FP64 per zone-update and 56 B of flags): the metric counted
+# once per (x1, x2) plane as this layout stores it (amortised over NZ = 64,
+# about 760 B), and counted per cell in 3-D as KORAL tabulates it (4288 B).
+GRMHD_BYTES_PER_ZONE = {"axisymmetric_metric": 760.0, "metric_3d": 4288.0}
 
 # Algorithmic FP64 operations per implicit solve of the seeded batch workload
 # (all six fallback attempts included), counted by the operation-counting
@@ -352,22 +369,24 @@ def cpu_baseline_implicit(P, pp, g20, G20, dt, args):
                       % (ns, threads)}
 
 
-# Algorithmic HBM bytes per thread-unit of each grid operator, counted as the
-# compulsory traffic of that operator with the metric as KORAL tabulates it
-# (20+20 doubles per cell or face side, 64 Christoffels per cell): every
-# primitive/conserved vector is one NV-vector of FP64, stencil neighbours
-# are shared through the caches.
-def grid_alg_bytes(kind, nv):
+# Algorithmic HBM bytes per thread-unit of each grid operator: the
+# compulsory traffic of that operator as this build lays the data out.  Each
+# primitive/conserved vector is one NV-vector of FP64; stencil neighbours are
+# shared through the caches; the metric lives in per-(x1, x2) tables read
+# through the caches by every x3 column (no per-unit HBM bytes).
+def grid_alg_bytes(kind, nv, fused=False):
     return {
         "recon": (nv + 2 * nv) * 8,                # per face: prims of its cell, both face states
-        "flux": (2 * nv + 6 + 40 + nv) * 8,        # per face: face states, 6 speeds, face metric, flux
-        "u2p": (4 * nv + 40) * 8 + 4,              # per cell: u, p in/out, metric, 4 flags
-        "update": (nv + nv + 3 * nv + 40 + 64 + nv) * 8,  # per cell: p, u, 3 face fluxes, metric, Gamma, u
-        "wavespeeds": (nv + 40 + 18) * 8,          # per cell: prims, metric, 18 speeds
+        # per face: face states (split) or cell prims (fused), 3 wave-speed
+        # values (each cell's shared by its two faces), the flux
+        "flux": ((nv if fused else 2 * nv) + 3 + nv) * 8,
+        "u2p": (4 * nv) * 8 + 4,                   # per cell: u, p in and out, 4 flag bytes
+        "update": (nv + nv + 3 * nv + nv) * 8,     # per cell: p, u, 3 face fluxes, u
+        "wavespeeds": (nv + 18) * 8,               # per cell: prims, 18 speeds
         "emf": 9 * 8, "ct_faces": 9 * 8,           # per edge
         "comb": 3 * 8,                             # per (cell, variable)
-        "bc_faces": (3 * nv + 40) * 8, "bc_edge": 4 * (3 * nv + 40) * 8, "bc_total": (3 * nv + 40) * 8,
-        "p2u": (2 * nv + 40) * 8,
+        "bc_faces": (3 * nv) * 8, "bc_edge": 4 * (3 * nv) * 8, "bc_total": (3 * nv) * 8,
+        "p2u": (2 * nv) * 8,
         "fixup": 8, "fixup_apply": 8,
         "implicit": (4 * nv + 40 + 1) * 8 + 3,
     }.get(kind, 0)
@@ -406,7 +425,8 @@ def bench_grid(args, kind):
         chosen = [r for r in rows if r[0].startswith("chose")]
         tune_info = {"seconds": time.perf_counter() - t0, "trials": len(rows) - len(chosen),
                      "start_ms_per_step": rows[0][2], "end_ms_per_step": chosen[-1][2] if chosen else rows[0][2],
-                     "fused": list(tn.fused), "floor_wave": tn.floor_wave, "floor_nw": tn.floor_nw,
+                     "fused": list(tn.fused), "floor_wave": tn.floor_wave, "update_pre": tn.update_pre,
+                     "floor_nw": tn.floor_nw,
                      "u2p_nw": tn.u2p_nw,
                      "block": {names[k]: tn.block[k] for k in range(32) if names[k] and tn.block[k] != 256}}
     # pilot: three steps with HIP events around every launch give the
@@ -443,29 +463,64 @@ def bench_grid(args, kind):
     dom = max((k for k in tm if not k.endswith("_floor")), key=lambda k: tm[k][0])
     ms, units, nl = tm[dom]
     per_launch = ms / nl / 1e3
-    bytes_launch = grid_alg_bytes(dom, P.nv) * units / nl
+    tn = dg.get_tune()
+    dims = [d for d in range(3) if n[d] > 1]
+    nfused = sum(1 for d in dims if tn.fused[d])
+    # face kernels: the bytes of each direction's schedule, averaged
+    if dom == "flux":
+        bpu = sum(grid_alg_bytes("flux", P.nv, bool(tn.fused[d])) for d in dims) / len(dims)
+    else:
+        bpu = grid_alg_bytes(dom, P.nv)
+    bytes_launch = bpu * units / nl
     achieved = bytes_launch / per_launch / 1e9
+    traffic = None
+    if kind == "grmhd" and n == (128, 128, 64):
+        key = dom if dom != "flux" else ("flux_split" if nfused == 0 else None)
+        traffic = GRID_TRAFFIC_128x128x64.get(key) if key else None
+    step_s = wall / args.steps
+    zups = zones * args.steps / wall
+    step_info = None
+    if kind == "grmhd":
+        fl = GRMHD_FLOPS_PER_ZONE["step"] * zups / 1e12
+        step_info = {"fp64_flops_per_zone_update": GRMHD_FLOPS_PER_ZONE["step"],
+                     "fp64_achieved_TFLOPs": fl, "fp64_frac": fl / FP64_PEAK_TFLOPS}
+        for k, b in GRMHD_BYTES_PER_ZONE.items():
+            step_info["hbm_bytes_per_zone_update_" + k] = b
+            step_info["hbm_frac_" + k] = b * zups / 1e9 / HBM_PEAK_GBS
+        # the face sweeps (reconstruction + fluxes) against their FP64 count
+        sw_ms = (split.get("flux", (0.,))[0] + split.get("recon", (0.,))[0]) / 3.
+        if sw_ms > 0:
+            sw = GRMHD_FLOPS_PER_ZONE["sweeps"] * zones / (sw_ms / 1e3) / 1e12
+            step_info["sweeps_ms_per_step"] = sw_ms
+            step_info["sweeps_fp64_TFLOPs"] = sw
+            step_info["sweeps_fp64_frac"] = sw / FP64_PEAK_TFLOPS
     res = {
         "metric": METRIC,
-        "value": zones * args.steps / wall,
+        "value": zups,
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": step_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded torus initial data)",
         "config": {"workload": label, "grid": list(n), "nv": P.nv, "ng": spec.ng, "autotune": tune_info,
+                   "headline_note": ("the GRRMHD+M1 step the metric names is refused by this build (unresolved "
+                                     "device fault of an earlier build); this line is the GRMHD step")
+                   if kind == "grmhd" else None,
                    "counters_per_step": (counters / args.steps).tolist(),
                    "operator_ms_per_step_pilot": {k: v[0] / 3 for k, v in sorted(split.items(), key=lambda kv: -kv[1][0])}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": GRID_TRAFFIC_128x128x64.get(dom) if (kind == "grmhd" and n == (128, 128, 64)) else None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_GBps": traffic / per_launch / 1e9 if traffic else None,
                      "kernel": dom, "kernel_ms": per_launch * 1e3, "launches": nl,
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "fused_sweeps": nfused, "algorithmic_bytes_per_launch": bytes_launch,
+                     "algorithmic_bytes_basis": "compulsory bytes of this layout (metric in per-(x1,x2) tables, "
+                                                "served by the caches)",
+                     "step": step_info},
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_grid(P, kind, args, n)

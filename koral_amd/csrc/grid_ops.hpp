@@ -801,10 +801,14 @@ struct OpCtFaces {
 // ---------------------------------------------------------------------------
 // conservative update with metric sources (finite.c:1062-1160)
 // ---------------------------------------------------------------------------
-template <int MAG, int RAD>
+// PRE: the cell's own u and p are loaded together with the fluxes, so their
+// loads are in flight at once instead of in three dependent rounds (more
+// registers: 3 waves per SIMD); otherwise p is loaded for the metric source
+// and u at the end.  The arithmetic of every term is the same either way.
+template <int MAG, int RAD, bool PRE = false>
 struct OpUpdate {
   static constexpr int kind = K_UPDATE;
-  static constexpr int waves_per_simd = RAD ? 3 : 4;
+  static constexpr int waves_per_simd = (RAD || PRE) ? 3 : 4;
   koral_params P;
   DGrid G;
   const double *p;
@@ -822,9 +826,12 @@ struct OpUpdate {
     long c = cidx(G, ix, iy, iz);
     // flux divergence first, metric source second: the compiler fence keeps
     // the 64 Christoffel and 40 metric loads from being hoisted beside the
-    // 6 NV flux loads (253 VGPRs, 2 waves per SIMD otherwise); the
-    // arithmetic of every term is unchanged
-    double du[NV];
+    // 6 NV flux loads (253 VGPRs, 2 waves per SIMD otherwise)
+    double du[NV], u0[NV], pp[NV];
+    if (PRE) {
+      load_p<MAG, RAD>(u, G.ncell, c, u0);
+      load_p<MAG, RAD>(p, G.ncell, c, pp);
+    }
     double dx = size_x(G, ix, 0), dy = size_x(G, iy, 1), dz = size_x(G, iz, 2);
     long fxl = fxidx(G, ix, iy, iz), fxr = fxidx(G, ix + 1, iy, iz);
     long fyl = 0, fyr = 0, fzl = 0, fzr = 0;
@@ -837,13 +844,16 @@ struct OpUpdate {
       if (G.nz > 1) du[iv] = du[iv] - (F.z[iv * F.nfz + fzr] - F.z[iv * F.nfz + fzl]) * dt / dz;
     }
     compiler_fence();
-    double pp[NV], ms[NV];
-    load_p<MAG, RAD>(p, G.ncell, c, pp);
+    if (!PRE) load_p<MAG, RAD>(p, G.ncell, c, pp);
+    double ms[NV];
     Geom g;
     geom_cell(G, ix, iy, g);
     metric_source<MAG, RAD>(P, G, pp, ix, iy, g, ms);
 #pragma unroll
-    for (int iv = 0; iv < NV; iv++) u[iv * G.ncell + c] = u[iv * G.ncell + c] + du[iv] + ms[iv] * dt;
+    for (int iv = 0; iv < NV; iv++) {
+      const double uo = PRE ? u0[iv] : u[iv * G.ncell + c];
+      u[iv * G.ncell + c] = uo + du[iv] + ms[iv] * dt;
+    }
   }
 };
 
@@ -1175,6 +1185,7 @@ struct GridTune {
   int fused[3];         // sweep of direction d: 1 reconstruction inside the flux kernel
                         // (OpSweep), 0 face states through HBM (OpRecon + OpFlux)
   int floor_wave;       // deferred passes: 1 one wave per face or cell, 0 one lane
+  int update_pre;       // update: 1 the cell's u and p loaded with the fluxes
   long floor_nw;        // waves (or lanes) in flight in the deferred face passes
   long u2p_nw;          // waves (or lanes) in flight in the deferred u2p pass
 };
@@ -1184,6 +1195,7 @@ inline GridTune default_tune()
   for (int k = 0; k < TUNE_KINDS; k++) t.block[k] = 256;
   for (int d = 0; d < 3; d++) t.fused[d] = 0;
   t.floor_wave = 1;
+  t.update_pre = 0;
   t.floor_nw = 1024;
   t.u2p_nw = 4096;
   return t;
@@ -1438,7 +1450,10 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false, bool co
   }
   dispatch(P, [&](auto t) {
     using T = decltype(t);
-    ex.each(OpUpdate<T::MAG, T::RAD>{P, G, s.p, s.u, F, s.clock});
+    if (s.tune.update_pre)
+      ex.each(OpUpdate<T::MAG, T::RAD, true>{P, G, s.p, s.u, F, s.clock});
+    else
+      ex.each(OpUpdate<T::MAG, T::RAD, false>{P, G, s.p, s.u, F, s.clock});
   });
 }
 

@@ -862,6 +862,7 @@ void tune_to_c(const GridTune &a, koral_grid_tune *t)
   for (int k = 0; k < TUNE_KINDS; k++) t->block[k] = a.block[k];
   for (int d = 0; d < 3; d++) t->fused[d] = a.fused[d];
   t->floor_wave = a.floor_wave;
+  t->update_pre = a.update_pre;
   t->floor_nw = a.floor_nw;
   t->u2p_nw = a.u2p_nw;
 }
@@ -876,6 +877,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
   for (int k = 0; k < TUNE_KINDS; k++) a.block[k] = t->block[k];
   for (int d = 0; d < 3; d++) a.fused[d] = t->fused[d] ? 1 : 0;
   a.floor_wave = t->floor_wave ? 1 : 0;
+  a.update_pre = t->update_pre ? 1 : 0;
   a.floor_nw = t->floor_nw;
   a.u2p_nw = t->u2p_nw;
   return 0;
@@ -1017,6 +1019,7 @@ int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, c
     coord("floor_nw", [&] { return T.floor_nw; }, [&](long v) { T.floor_nw = v; }, {256, 1024, 4096});
   }
   coord("u2p_nw", [&] { return T.u2p_nw; }, [&](long v) { T.u2p_nw = v; }, {1024, 4096, 16384});
+  coord("update_pre", [&] { return (long)T.update_pre; }, [&](long v) { T.update_pre = (int)v; }, {0, 1});
   // block sizes of the operator kinds a step launches
   const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
                        K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};

@@ -348,6 +348,7 @@ def test_grid_launch_choices_bit_identical(kind):
         for d in range(3):
             t.fused[d] = fused
         t.floor_wave = wave
+        t.update_pre = fused
         for k in range(32):
             t.block[k] = block
         t.floor_nw, t.u2p_nw = floor_nw, u2p_nw
