@@ -426,6 +426,7 @@ def bench_grid(args, kind):
         tune_info = {"seconds": time.perf_counter() - t0, "trials": len(rows) - len(chosen),
                      "start_ms_per_step": rows[0][2], "end_ms_per_step": chosen[-1][2] if chosen else rows[0][2],
                      "fused": list(tn.fused), "floor_wave": tn.floor_wave, "update_pre": tn.update_pre,
+                     "comb_nv": tn.comb_nv,
                      "floor_nw": tn.floor_nw,
                      "u2p_nw": tn.u2p_nw,
                      "block": {names[k]: tn.block[k] for k in range(32) if names[k] and tn.block[k] != 256}}

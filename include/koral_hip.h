@@ -340,6 +340,8 @@ typedef struct koral_grid_tune {
   int fused[3];
   int floor_wave;
   int update_pre;   /* update kernel: 1 loads the cell's u and p with the fluxes */
+  int comb_nv;        /* 1: the stage-algebra kernel is compiled for the
+                         grid's variable count */
   long floor_nw, u2p_nw;
 } koral_grid_tune;
 int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t);

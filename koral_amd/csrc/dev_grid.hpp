@@ -76,10 +76,40 @@ KD_INL double size_x(const DGrid &G, int ic, int d)
   return xb[ic + 1 + G.ng] - xb[ic + G.ng];
 }
 
+// SoA element x[k + c] for a wave-uniform k (a variable's offset k = iv *
+// stride) and a per-lane index c, with cb = 8 c.  On the device it is a
+// buffer access: the base x + k forms a buffer resource in scalar registers
+// and the 32-bit byte offset cb is the only vector operand, shared by every
+// variable of the cell (a global access would carry a 64-bit address per
+// load in vector registers).  Needs 8 c < 2^32 (the grid checks that every
+// variable has fewer than 2^29 cells and faces, which also bounds the
+// metric tables).
+typedef unsigned kd_u32x2 __attribute__((ext_vector_type(2)));
+KD_INL unsigned soa_off(long c) { return (unsigned)c * 8u; }
+#if defined(__HIP_DEVICE_COMPILE__)
+KD_INL __amdgpu_buffer_rsrc_t soa_rsrc(const double *x, long k)
+{
+  // raw buffer, no stride, the whole 32-bit range, 32-bit data format
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(x + k), (short)0, (int)0xffffffff, 0x00020000);
+}
+KD_INL double soa_ld(const double *x, long k, unsigned cb)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(soa_rsrc(x, k), cb, 0, 0));
+}
+KD_INL void soa_st(double *x, long k, unsigned cb, double v)
+{
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(kd_u32x2, v), soa_rsrc(x, k), cb, 0, 0);
+}
+#else
+KD_INL double soa_ld(const double *x, long k, unsigned cb) { return *(const double *)((const char *)(x + k) + cb); }
+KD_INL void soa_st(double *x, long k, unsigned cb, double v) { *(double *)((char *)(x + k) + cb) = v; }
+#endif
+
 KD_INL void geom_cell(const DGrid &G, int ix, int iy, Geom &g)
 {
   const long m = midx(G, ix, iy), n = G.npl;
-  load_geom(g, [&](int c) { return G.gc[c * n + m]; }, [&](int c) { return G.Gc[c * n + m]; });
+  const unsigned mb = soa_off(m);
+  load_geom(g, [&](int c) { return soa_ld(G.gc, c * n, mb); }, [&](int c) { return soa_ld(G.Gc, c * n, mb); });
 }
 
 KD_INL void geom_face(const DGrid &G, int d, int ix, int iy, Geom &g)
@@ -87,11 +117,13 @@ KD_INL void geom_face(const DGrid &G, int d, int ix, int iy, Geom &g)
   if (d == 0) {
     KD_CHECK(in_faces(ix, G.nx, G.ngx) && in_cells(iy, G.ny, G.ngy), "geom_face x", ix, iy, 0);
     const long m = (long)(G.ny > 1 ? iy + G.ngy : 0) * (G.sx + 1) + ix + G.ngx, n = G.nplx;
-    load_geom(g, [&](int c) { return G.gfx[c * n + m]; }, [&](int c) { return G.Gfx[c * n + m]; });
+    const unsigned mb = soa_off(m);
+    load_geom(g, [&](int c) { return soa_ld(G.gfx, c * n, mb); }, [&](int c) { return soa_ld(G.Gfx, c * n, mb); });
   } else if (d == 1) {
     KD_CHECK(G.ny > 1 && in_cells(ix, G.nx, G.ngx) && in_faces(iy, G.ny, G.ngy), "geom_face y", ix, iy, 0);
     const long m = (long)(iy + G.ngy) * G.sx + ix + G.ngx, n = G.nply;
-    load_geom(g, [&](int c) { return G.gfy[c * n + m]; }, [&](int c) { return G.Gfy[c * n + m]; });
+    const unsigned mb = soa_off(m);
+    load_geom(g, [&](int c) { return soa_ld(G.gfy, c * n, mb); }, [&](int c) { return soa_ld(G.Gfy, c * n, mb); });
   } else {
     geom_cell(G, ix, iy, g);
   }
@@ -369,6 +401,7 @@ KD_INL void metric_source(const koral_params &P, const DGrid &G, const double *p
   constexpr int NV = Lay<MAG, RAD>::NV;
   constexpr int EE = Lay<MAG, RAD>::EE;
   const long m = midx(G, ix, iy), npl = G.npl;
+  const unsigned mb = soa_off(m);
   double gdetu = g.gdet;
   double ucon[4], ucov[4], bcon[4] = {0., 0., 0., 0.}, bsq = 0.;
   relr_to_4vel(pp[2], pp[3], pp[4], ucon, ucov, g);
@@ -415,7 +448,7 @@ KD_INL void metric_source(const koral_params &P, const DGrid &G, const double *p
     for (int l = 0; l < 4; l++) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        double K = G.kr[(long)(l * 16 + j * 4 + k) * npl + m];
+        double K = soa_ld(G.kr, (long)(l * 16 + j * 4 + k) * npl, mb);
         ss[1 + j] += gdetu * Tm[l] * K;
         if (RAD) ss[EE + j] += gdetu * Rm[l] * K;
       }

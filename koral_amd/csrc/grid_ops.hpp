@@ -48,14 +48,16 @@ struct Cnt3 {
 template <int MAG, int RAD>
 KD_INL void load_p(const double *p, long n, long c, double *v)
 {
+  const unsigned cb = soa_off(c);
 #pragma unroll
-  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) v[iv] = p[iv * n + c];
+  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) v[iv] = soa_ld(p, iv * n, cb);
 }
 template <int MAG, int RAD>
 KD_INL void store_p(double *p, long n, long c, const double *v)
 {
+  const unsigned cb = soa_off(c);
 #pragma unroll
-  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) p[iv * n + c] = v[iv];
+  for (int iv = 0; iv < Lay<MAG, RAD>::NV; iv++) soa_st(p, iv * n, cb, v[iv]);
 }
 
 template <int MAG, int RAD>
@@ -798,6 +800,159 @@ struct OpCtFaces {
   }
 };
 
+// A coefficient of the step algebra as the expression of ko.c:444-570 it
+// stands for, evaluated on the device from the clock's dt with the same
+// operations the host would use: c, c / (dt * gamma), c / dt, dt * c, dt / c.
+struct Coef {
+  enum Kind { CONST, C_OVER_DTG, C_OVER_DT, DT_TIMES_C, DT_OVER_C } kind;
+  double c;
+  KD_HD double eval(double dt, double gamma) const
+  {
+    switch (kind) {
+    case C_OVER_DTG: return c / (dt * gamma);
+    case C_OVER_DT: return c / dt;
+    case DT_TIMES_C: return dt * c;
+    case DT_OVER_C: return dt / c;
+    default: return c;
+    }
+  }
+};
+
+// RK2IMEX stage algebra (ko.c:444-570): up to three statements out = a x
+// [+ b y [+ c z]], evaluated left to right, per variable of one interior
+// cell, each term an array or the result of an earlier statement of the
+// same launch.  A statement's arithmetic does not depend on what it is
+// fused with, so fusing gives the same bits as separate launches; each
+// input array is read once per launch.
+struct CombTerm {
+  Coef c;
+  const double *x;  // the array, or null for the result of statement prev
+  int prev;
+};
+struct CombStmt {
+  double *out;  // null: the result only feeds later statements
+  int nt;
+  CombTerm t[3];
+};
+struct OpCombSeq {
+  static constexpr int kind = K_COMB;
+  DGrid G;
+  int nv;
+  int ns;
+  CombStmt st[3];
+  const double *clk;
+  double gamma;
+  unsigned *zero = nullptr;  // counters cleared by the first nzero items
+  int nzero = 0;
+  // one interior cell per item, all its variables
+  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
+  KD_INL void coefs(double coef[3][3]) const
+  {
+    const double dt = clk[1];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) coef[q][j] = st[q].t[j].c.eval(dt, gamma);
+  }
+  // the statements for variable entry k
+  KD_INL void var(long k, const double coef[3][3]) const
+  {
+    double r0 = 0., r1 = 0., r2 = 0.;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (q >= ns) break;
+      const CombStmt &S = st[q];
+      double v[3] = {0., 0., 0.};
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        if (j < S.nt) {
+          const int pq = S.t[j].prev;
+          const double *x = S.t[j].x;
+          v[j] = x ? x[k] : (pq == 0 ? r0 : (pq == 1 ? r1 : r2));
+        }
+      const double a = coef[q][0], b = coef[q][1], cc = coef[q][2];
+      double w;
+      if (S.nt == 3)
+        w = a * v[0] + b * v[1] + cc * v[2];
+      else if (S.nt == 2)
+        w = a * v[0] + b * v[1];
+      else
+        w = a * v[0];
+      if (q == 0) r0 = w;
+      if (q == 1) r1 = w;
+      if (q == 2) r2 = w;
+      if (S.out) S.out[k] = w;
+    }
+  }
+  // the statements for the NV variables of interior cell c, statement by
+  // statement: every load of a statement is issued before its stores, so a
+  // statement costs one memory round trip however many variables and terms
+  // it has (the stores may alias later loads, so per variable they would
+  // not overlap).  The sum a x + b y + c z is accumulated left to right,
+  // the same operations in the same order as var().  No statement refers
+  // to the result of the third (the last), so two results are kept.
+  template <int NV>
+  KD_INL void cell_vars(long c, const double coef[3][3]) const
+  {
+    const unsigned cb = soa_off(c);
+    double r0[NV], r1[NV];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (q >= ns) break;
+      const CombStmt &S = st[q];
+      double w[NV];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        if (j >= S.nt) break;
+        const double *x = S.t[j].x;
+        const int pq = S.t[j].prev;
+        const double a = coef[q][j];
+        double v[NV];
+        if (x) {
+#pragma unroll
+          for (int iv = 0; iv < NV; iv++) v[iv] = soa_ld(x, iv * G.ncell, cb);
+        } else {
+#pragma unroll
+          for (int iv = 0; iv < NV; iv++) v[iv] = pq == 0 ? r0[iv] : r1[iv];
+        }
+#pragma unroll
+        for (int iv = 0; iv < NV; iv++) w[iv] = j == 0 ? a * v[iv] : w[iv] + a * v[iv];
+      }
+      if (S.out)
+#pragma unroll
+        for (int iv = 0; iv < NV; iv++) soa_st(S.out, iv * G.ncell, cb, w[iv]);
+#pragma unroll
+      for (int iv = 0; iv < NV; iv++) {
+        if (q == 0) r0[iv] = w[iv];
+        if (q == 1) r1[iv] = w[iv];
+      }
+    }
+  }
+  KD_INL void operator()(long i) const
+  {
+    if (i < nzero) zero[i] = 0u;
+    const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
+    const long c = cidx(G, ix, iy, iz);
+    double coef[3][3];
+    coefs(coef);
+    for (int iv = 0; iv < nv; iv++) var((long)iv * G.ncell + c, coef);
+  }
+};
+
+// OpCombSeq with the variable count fixed at compile time: each statement's
+// loads for all NV variables are in flight together (cell_vars)
+template <int NV>
+struct OpCombSeqN : OpCombSeq {
+  KD_INL void operator()(long i) const
+  {
+    if (i < nzero) zero[i] = 0u;
+    const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
+    double coef[3][3];
+    coefs(coef);
+    this->template cell_vars<NV>(cidx(G, ix, iy, iz), coef);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // conservative update with metric sources (finite.c:1062-1160)
 // ---------------------------------------------------------------------------
@@ -837,11 +992,13 @@ struct OpUpdate {
     long fyl = 0, fyr = 0, fzl = 0, fzr = 0;
     if (G.ny > 1) { fyl = fyidx(G, ix, iy, iz); fyr = fyidx(G, ix, iy + 1, iz); }
     if (G.nz > 1) { fzl = fzidx(G, ix, iy, iz); fzr = fzidx(G, ix, iy, iz + 1); }
+    const unsigned bxl = soa_off(fxl), bxr = soa_off(fxr), byl = soa_off(fyl), byr = soa_off(fyr),
+                   bzl = soa_off(fzl), bzr = soa_off(fzr);
 #pragma unroll
     for (int iv = 0; iv < NV; iv++) {
-      du[iv] = -(F.x[iv * F.nfx + fxr] - F.x[iv * F.nfx + fxl]) * dt / dx;
-      if (G.ny > 1) du[iv] = du[iv] - (F.y[iv * F.nfy + fyr] - F.y[iv * F.nfy + fyl]) * dt / dy;
-      if (G.nz > 1) du[iv] = du[iv] - (F.z[iv * F.nfz + fzr] - F.z[iv * F.nfz + fzl]) * dt / dz;
+      du[iv] = -(soa_ld(F.x, iv * F.nfx, bxr) - soa_ld(F.x, iv * F.nfx, bxl)) * dt / dx;
+      if (G.ny > 1) du[iv] = du[iv] - (soa_ld(F.y, iv * F.nfy, byr) - soa_ld(F.y, iv * F.nfy, byl)) * dt / dy;
+      if (G.nz > 1) du[iv] = du[iv] - (soa_ld(F.z, iv * F.nfz, bzr) - soa_ld(F.z, iv * F.nfz, bzl)) * dt / dz;
     }
     compiler_fence();
     if (!PRE) load_p<MAG, RAD>(p, G.ncell, c, pp);
@@ -849,10 +1006,13 @@ struct OpUpdate {
     Geom g;
     geom_cell(G, ix, iy, g);
     metric_source<MAG, RAD>(P, G, pp, ix, iy, g, ms);
+    {
+      const unsigned cb = soa_off(c);
 #pragma unroll
-    for (int iv = 0; iv < NV; iv++) {
-      const double uo = PRE ? u0[iv] : u[iv * G.ncell + c];
-      u[iv * G.ncell + c] = uo + du[iv] + ms[iv] * dt;
+      for (int iv = 0; iv < NV; iv++) {
+        const double uo = PRE ? u0[iv] : soa_ld(u, iv * G.ncell, cb);
+        soa_st(u, iv * G.ncell, cb, uo + du[iv] + ms[iv] * dt);
+      }
     }
   }
 };
@@ -1082,94 +1242,6 @@ struct OpImplicit {
   }
 };
 
-// A coefficient of the step algebra as the expression of ko.c:444-570 it
-// stands for, evaluated on the device from the clock's dt with the same
-// operations the host would use: c, c / (dt * gamma), c / dt, dt * c, dt / c.
-struct Coef {
-  enum Kind { CONST, C_OVER_DTG, C_OVER_DT, DT_TIMES_C, DT_OVER_C } kind;
-  double c;
-  KD_HD double eval(double dt, double gamma) const
-  {
-    switch (kind) {
-    case C_OVER_DTG: return c / (dt * gamma);
-    case C_OVER_DT: return c / dt;
-    case DT_TIMES_C: return dt * c;
-    case DT_OVER_C: return dt / c;
-    default: return c;
-    }
-  }
-};
-
-// RK2IMEX stage algebra (ko.c:444-570): up to three statements out = a x
-// [+ b y [+ c z]], evaluated left to right, per variable of one interior
-// cell, each term an array or the result of an earlier statement of the
-// same launch.  A statement's arithmetic does not depend on what it is
-// fused with, so fusing gives the same bits as separate launches; each
-// input array is read once per launch.
-struct CombTerm {
-  Coef c;
-  const double *x;  // the array, or null for the result of statement prev
-  int prev;
-};
-struct CombStmt {
-  double *out;  // null: the result only feeds later statements
-  int nt;
-  CombTerm t[3];
-};
-struct OpCombSeq {
-  static constexpr int kind = K_COMB;
-  DGrid G;
-  int nv;
-  int ns;
-  CombStmt st[3];
-  const double *clk;
-  double gamma;
-  unsigned *zero = nullptr;  // counters cleared by the first nzero items
-  int nzero = 0;
-  // one interior cell per item, all its variables
-  KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
-  KD_INL void operator()(long i) const
-  {
-    if (i < nzero) zero[i] = 0u;
-    const double dt = clk[1];
-    const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
-    const long c = cidx(G, ix, iy, iz);
-    double coef[3][3];
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) coef[q][j] = st[q].t[j].c.eval(dt, gamma);
-    for (int iv = 0; iv < nv; iv++) {
-      const long k = (long)iv * G.ncell + c;
-      double r0 = 0., r1 = 0., r2 = 0.;
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        if (q >= ns) break;
-        const CombStmt &S = st[q];
-        double v[3] = {0., 0., 0.};
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          if (j < S.nt) {
-            const int pq = S.t[j].prev;
-            v[j] = S.t[j].x ? S.t[j].x[k] : (pq == 0 ? r0 : (pq == 1 ? r1 : r2));
-          }
-        const double a = coef[q][0], b = coef[q][1], cc = coef[q][2];
-        double w;
-        if (S.nt == 3)
-          w = a * v[0] + b * v[1] + cc * v[2];
-        else if (S.nt == 2)
-          w = a * v[0] + b * v[1];
-        else
-          w = a * v[0];
-        if (q == 0) r0 = w;
-        if (q == 1) r1 = w;
-        if (q == 2) r2 = w;
-        if (S.out) S.out[k] = w;
-      }
-    }
-  }
-};
-
 // ---------------------------------------------------------------------------
 // step order, written once for any executor
 // ---------------------------------------------------------------------------
@@ -1186,6 +1258,7 @@ struct GridTune {
                         // (OpSweep), 0 face states through HBM (OpRecon + OpFlux)
   int floor_wave;       // deferred passes: 1 one wave per face or cell, 0 one lane
   int update_pre;       // update: 1 the cell's u and p loaded with the fluxes
+  int comb_nv;          // 1: stage algebra with the variable count a template parameter
   long floor_nw;        // waves (or lanes) in flight in the deferred face passes
   long u2p_nw;          // waves (or lanes) in flight in the deferred u2p pass
 };
@@ -1196,6 +1269,7 @@ inline GridTune default_tune()
   for (int d = 0; d < 3; d++) t.fused[d] = 0;
   t.floor_wave = 1;
   t.update_pre = 0;
+  t.comb_nv = 1;
   t.floor_nw = 1024;
   t.u2p_nw = 4096;
   return t;
@@ -1375,6 +1449,10 @@ void explicit_op(Ex &ex, const GridBufs &s, bool ghosts_current = false, bool co
   const int tt = P.mag ? 1 : 0;
   const int n[3] = {G.nx, G.ny, G.nz};
   if (!counts_cleared) ex.each(OpZeroU32{s.fcnt, 3});
+  // counts_cleared: the caller's last launch cleared the face-list counts
+  // (the stage algebra's first items do); the host check build verifies it
+  KD_CHECK(!counts_cleared || (s.fcnt[0] == 0u && s.fcnt[1] == 0u && s.fcnt[2] == 0u),
+           "explicit_op: face-list counts not cleared", s.fcnt[0], s.fcnt[1], s.fcnt[2]);
   // x2 first: its floor pass (faces next to the magnetised torus surface,
   // the longest) then overlaps the x1 and x3 sweeps.  The sweeps write
   // separate flux arrays, so their order does not change any result.
@@ -1479,6 +1557,7 @@ void u2p(Ex &ex, const GridBufs &s, bool count_cleared = false)
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
     if (!count_cleared) ex.each(OpZeroU32{s.fcnt + 3, 1});
+    KD_CHECK(!count_cleared || s.fcnt[3] == 0u, "u2p: cell-list count not cleared", s.fcnt[3], 0, 0);
     const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy};
     ex.count(op, s.counters + 3);
     if (s.tune.floor_wave)
@@ -1573,6 +1652,20 @@ void implicit_op(Ex &ex, const GridBufs &s, double fac)
   fixup(ex, s, F_RADIMPFIXUP);
 }
 
+// launches a stage-algebra op (compile-time variable count when tune.comb_nv)
+template <class Ex>
+void each_comb(Ex &ex, const GridBufs &s, const OpCombSeq &op)
+{
+  if (!s.tune.comb_nv) {
+    ex.each(op);
+    return;
+  }
+  dispatch(s.P, [&](auto t) {
+    using T = decltype(t);
+    ex.each(OpCombSeqN<Lay<T::MAG, T::RAD>::NV>{op});
+  });
+}
+
 template <class Ex>
 void comb(Ex &ex, const GridBufs &s, double gamma, double *out, Coef a, const double *x,
           Coef b = {Coef::CONST, 0.}, const double *y = nullptr, Coef c = {Coef::CONST, 0.},
@@ -1581,7 +1674,7 @@ void comb(Ex &ex, const GridBufs &s, double gamma, double *out, Coef a, const do
   const int nt = z ? 3 : (y ? 2 : 1);
   OpCombSeq op{s.G, s.P.nv, 1, {}, s.clock, gamma};
   op.st[0] = CombStmt{out, nt, {CombTerm{a, x, 0}, CombTerm{b, y, 0}, CombTerm{c, z, 0}}};
-  ex.each(op);
+  each_comb(ex, s, op);
 }
 
 // body of one RK2IMEX step, ko.c:444-570, at the clock's (t, dt); the
@@ -1601,30 +1694,46 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   // each seq() precedes a u2p and also clears the four list counters of the
   // u2p and of the explicit operator that follows it (no reader of them runs
   // in between: the floor passes of the previous sweeps were joined)
-  auto seq = [&](std::initializer_list<CombStmt> l) {
+  // the statements of one launch; with `launch` false the op is returned
+  // for the update to evaluate (the counters are then cleared by it)
+  auto make = [&](std::initializer_list<CombStmt> l) {
     OpCombSeq op{s.G, s.P.nv, (int)l.size(), {}, s.clock, gamma};
     int q = 0;
     for (const CombStmt &st : l) op.st[q++] = st;
     if (op.size() >= 4) {
       op.zero = s.fcnt;
       op.nzero = 4;
-    } else {
-      ex.each(OpZeroU32{s.fcnt, 4});
     }
-    ex.each(op);
+    return op;
   };
-  comb(ex, s, gamma, s.ut0, one, s.u);
-  implicit_op(ex, s, gamma);
-  // drt1 = (u - ut0) / (gamma dt); ut1 = u
-  seq({CombStmt{s.drt1, 2, {A(idtg, s.u), A(midtg, s.ut0), none}}, CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
+  auto seq = [&](std::initializer_list<CombStmt> l) {
+    const OpCombSeq op = make(l);
+    if (op.nzero == 0) ex.each(OpZeroU32{s.fcnt, 4});
+    each_comb(ex, s, op);
+  };
+  auto explicit_then = [&](std::initializer_list<CombStmt> l) {
+    // u2p ended with set_bc; without a polar correction p and u are unchanged
+    explicit_op(ex, s, !polar_correction_active(s), true);
+    seq(l);
+  };
+  if (!s.P.rad) {
+    // no implicit source between them (implicit_op is empty): ut0 = u;
+    // drt1 = (u - ut0) / (gamma dt) with ut0 as just computed; ut1 = u
+    seq({CombStmt{s.ut0, 1, {A(one, s.u), none, none}},
+         CombStmt{s.drt1, 2, {A(idtg, s.u), R(midtg, 0), none}},
+         CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
+  } else {
+    comb(ex, s, gamma, s.ut0, one, s.u);
+    implicit_op(ex, s, gamma);
+    // drt1 = (u - ut0) / (gamma dt); ut1 = u
+    seq({CombStmt{s.drt1, 2, {A(idtg, s.u), A(midtg, s.ut0), none}}, CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
+  }
   u2p(ex, s, true);
   do_correct(ex, s);
-  // u2p ended with set_bc; without a polar correction p and u are unchanged
-  explicit_op(ex, s, !polar_correction_active(s), true);
-  // dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
-  seq({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
-       CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
-       CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
+  // explicit; dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
+  explicit_then({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+                 CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
+                 CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
   u2p(ex, s, true);
   do_correct(ex, s);
   implicit_op(ex, s, gamma);
@@ -1633,12 +1742,10 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
        CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
   u2p(ex, s, true);
   do_correct(ex, s);
-  // u2p ended with set_bc; without a polar correction p and u are unchanged
-  explicit_op(ex, s, !polar_correction_active(s), true);
-  // dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
-  seq({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
-       CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
-       CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
+  // explicit; dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
+  explicit_then({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+                 CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
+                 CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
   u2p(ex, s, true);
   do_correct(ex, s);
 }

@@ -434,6 +434,12 @@ int koral_hip_grid_create(const koral_params *P, const koral_grid_desc *D, koral
   if ((P->int_order == 4 && D->ng < 4) || (P->int_order == 2 && D->ng < 3))
     return set_error(KORAL_E_ARG, "grid_create: NG too small for INT_ORDER");
   if (D->ny == 1 && D->nz > 1) return set_error(KORAL_E_ARG, "grid_create: NY=1 with NZ>1 not supported");
+  {
+    // the kernels address a variable's cells and faces with 32-bit byte offsets
+    const long sx = D->nx + 2L * D->ng, sy = D->ny > 1 ? D->ny + 2L * D->ng : 1, sz = D->nz > 1 ? D->nz + 2L * D->ng : 1;
+    if ((sx + 1) * (sy + 1) * (sz + 1) >= (1L << 29))
+      return set_error(KORAL_E_ARG, "grid_create: more than 2^29 cells or faces per variable (ghosts included)");
+  }
   if (P->polar_nc < 0 || (P->polar_nc > 0 && D->ny > 1 && D->ny < 2 * P->polar_nc + 2))
     return set_error(KORAL_E_ARG, "grid_create: polar-axis correction needs NY >= 2 NCCORRECTPOLAR + 2");
   for (int k = 0; k < 3; k++)
@@ -863,6 +869,7 @@ void tune_to_c(const GridTune &a, koral_grid_tune *t)
   for (int d = 0; d < 3; d++) t->fused[d] = a.fused[d];
   t->floor_wave = a.floor_wave;
   t->update_pre = a.update_pre;
+  t->comb_nv = a.comb_nv;
   t->floor_nw = a.floor_nw;
   t->u2p_nw = a.u2p_nw;
 }
@@ -878,6 +885,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
   for (int d = 0; d < 3; d++) a.fused[d] = t->fused[d] ? 1 : 0;
   a.floor_wave = t->floor_wave ? 1 : 0;
   a.update_pre = t->update_pre ? 1 : 0;
+  a.comb_nv = t->comb_nv ? 1 : 0;
   a.floor_nw = t->floor_nw;
   a.u2p_nw = t->u2p_nw;
   return 0;
@@ -1020,6 +1028,7 @@ int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, c
   }
   coord("u2p_nw", [&] { return T.u2p_nw; }, [&](long v) { T.u2p_nw = v; }, {1024, 4096, 16384});
   coord("update_pre", [&] { return (long)T.update_pre; }, [&](long v) { T.update_pre = (int)v; }, {0, 1});
+  coord("comb_nv", [&] { return (long)T.comb_nv; }, [&](long v) { T.comb_nv = (int)v; }, {1, 0});
   // block sizes of the operator kinds a step launches
   const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
                        K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};
