@@ -30,12 +30,17 @@ METRIC = "zone-updates/s (3D GRRMHD+M1, FP64); implicit solves/s learned vs plai
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
-# HBM bytes per launch of the grid kernels on the 128x128x64 torus, per
-# operator kind and schedule of its sweeps (keys: kind, or kind + "_fused" /
-# "_split" for the face kernels, averaged over the directions run that way):
+# HBM bytes per launch of the grid kernels on the 128x128x64 torus:
 # FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters on a
-# plain copy of known size (profiles/hbm_traffic_grmhd_torus_128x128x64.md).
-GRID_TRAFFIC_128x128x64 = {"u2p": 315e6, "flux_split": 445e6, "recon": 323e6, "update": 564e6}
+# streaming kernel of known size (profiles/hbm_traffic_grmhd_torus_128x128x64.md).
+# The face kernels per sweep direction (x1, x2, x3) and schedule (None: not
+# measured); the update per variant (tune.update_pre).
+GRID_TRAFFIC_128x128x64 = {
+    "u2p": 314e6,
+    "update": {1: 587e6},
+    "flux": {"split": (469e6, None, 307e6), "fused": (None, 791e6, None)},
+    "recon": {"split": (245e6, None, 357e6)},
+}
 
 # FP64 operations per zone-update of the GRMHD RK2IMEX step on the
 # 128x128x64 torus, counted by the operation-counting build of the oracle
@@ -417,7 +422,21 @@ def bench_grid(args, kind):
     # launch choices (block sizes, fused or split sweeps, deferred-pass
     # mapping) searched on this state before the timed steps
     tune_info = None
-    if not args.no_autotune:
+    if args.tune_json:
+        import json
+        want = json.loads(args.tune_json)
+        tn = dg.get_tune()
+        names = {dg.L.koral_hip_grid_op_name(k).decode(): k for k in range(32)}
+        for d, v in enumerate(want.get("fused", list(tn.fused))):
+            tn.fused[d] = int(v)
+        for key in ("floor_wave", "update_pre", "comb_nv", "floor_nw", "u2p_nw"):
+            if key in want:
+                setattr(tn, key, int(want[key]))
+        for nm, b in want.get("block", {}).items():
+            tn.block[names[nm]] = int(b)
+        dg.set_tune(tn)
+        tune_info = {"given": want}
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         rows = dg.autotune(steps=args.tune_steps)
         tn = dg.get_tune()
@@ -476,8 +495,14 @@ def bench_grid(args, kind):
     achieved = bytes_launch / per_launch / 1e9
     traffic = None
     if kind == "grmhd" and n == (128, 128, 64):
-        key = dom if dom != "flux" else ("flux_split" if nfused == 0 else None)
-        traffic = GRID_TRAFFIC_128x128x64.get(key) if key else None
+        tr = GRID_TRAFFIC_128x128x64.get(dom)
+        if dom == "flux":
+            v = [tr["fused" if tn.fused[d] else "split"][d] for d in dims]
+            traffic = sum(v) / len(v) if None not in v else None
+        elif dom == "update":
+            traffic = tr.get(tn.update_pre)
+        elif isinstance(tr, float):
+            traffic = tr
     step_s = wall / args.steps
     zups = zones * args.steps / wall
     step_info = None
@@ -601,6 +626,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
+    ap.add_argument("--tune-json", default=None,
+                    help="launch choices to use instead of the search, as the line's config.autotune reports "
+                         "them (fused, floor_wave, update_pre, comb_nv, floor_nw, u2p_nw, block)")
     ap.add_argument("--tune-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -818,12 +818,12 @@ struct Coef {
   }
 };
 
-// RK2IMEX stage algebra (ko.c:444-570): up to three statements out = a x
+// RK2IMEX stage algebra (ko.c:444-570): up to four statements out = a x
 // [+ b y [+ c z]], evaluated left to right, per variable of one interior
 // cell, each term an array or the result of an earlier statement of the
 // same launch.  A statement's arithmetic does not depend on what it is
-// fused with, so fusing gives the same bits as separate launches; each
-// input array is read once per launch.
+// fused with, so fusing gives the same bits as separate launches.
+constexpr int COMB_MAXST = 4;
 struct CombTerm {
   Coef c;
   const double *x;  // the array, or null for the result of statement prev
@@ -839,27 +839,27 @@ struct OpCombSeq {
   DGrid G;
   int nv;
   int ns;
-  CombStmt st[3];
+  CombStmt st[COMB_MAXST];
   const double *clk;
   double gamma;
   unsigned *zero = nullptr;  // counters cleared by the first nzero items
   int nzero = 0;
   // one interior cell per item, all its variables
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
-  KD_INL void coefs(double coef[3][3]) const
+  KD_INL void coefs(double coef[COMB_MAXST][3]) const
   {
     const double dt = clk[1];
 #pragma unroll
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < COMB_MAXST; q++)
 #pragma unroll
       for (int j = 0; j < 3; j++) coef[q][j] = st[q].t[j].c.eval(dt, gamma);
   }
   // the statements for variable entry k
-  KD_INL void var(long k, const double coef[3][3]) const
+  KD_INL void var(long k, const double coef[COMB_MAXST][3]) const
   {
-    double r0 = 0., r1 = 0., r2 = 0.;
+    double r[COMB_MAXST] = {0., 0., 0., 0.};
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < COMB_MAXST; q++) {
       if (q >= ns) break;
       const CombStmt &S = st[q];
       double v[3] = {0., 0., 0.};
@@ -868,7 +868,11 @@ struct OpCombSeq {
         if (j < S.nt) {
           const int pq = S.t[j].prev;
           const double *x = S.t[j].x;
-          v[j] = x ? x[k] : (pq == 0 ? r0 : (pq == 1 ? r1 : r2));
+          double rp = r[0];
+#pragma unroll
+          for (int m = 1; m < COMB_MAXST - 1; m++)
+            if (pq == m) rp = r[m];
+          v[j] = x ? x[k] : rp;
         }
       const double a = coef[q][0], b = coef[q][1], cc = coef[q][2];
       double w;
@@ -878,9 +882,7 @@ struct OpCombSeq {
         w = a * v[0] + b * v[1];
       else
         w = a * v[0];
-      if (q == 0) r0 = w;
-      if (q == 1) r1 = w;
-      if (q == 2) r2 = w;
+      r[q] = w;
       if (S.out) S.out[k] = w;
     }
   }
@@ -890,14 +892,14 @@ struct OpCombSeq {
   // it has (the stores may alias later loads, so per variable they would
   // not overlap).  The sum a x + b y + c z is accumulated left to right,
   // the same operations in the same order as var().  No statement refers
-  // to the result of the third (the last), so two results are kept.
+  // to the result of the last, so COMB_MAXST - 1 results are kept.
   template <int NV>
-  KD_INL void cell_vars(long c, const double coef[3][3]) const
+  KD_INL void cell_vars(long c, const double coef[COMB_MAXST][3]) const
   {
     const unsigned cb = soa_off(c);
-    double r0[NV], r1[NV];
+    double r[COMB_MAXST - 1][NV];
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < COMB_MAXST; q++) {
       if (q >= ns) break;
       const CombStmt &S = st[q];
       double w[NV];
@@ -913,7 +915,12 @@ struct OpCombSeq {
           for (int iv = 0; iv < NV; iv++) v[iv] = soa_ld(x, iv * G.ncell, cb);
         } else {
 #pragma unroll
-          for (int iv = 0; iv < NV; iv++) v[iv] = pq == 0 ? r0[iv] : r1[iv];
+          for (int iv = 0; iv < NV; iv++) {
+            v[iv] = r[0][iv];
+#pragma unroll
+            for (int m = 1; m < COMB_MAXST - 1; m++)
+              if (pq == m) v[iv] = r[m][iv];
+          }
         }
 #pragma unroll
         for (int iv = 0; iv < NV; iv++) w[iv] = j == 0 ? a * v[iv] : w[iv] + a * v[iv];
@@ -921,11 +928,9 @@ struct OpCombSeq {
       if (S.out)
 #pragma unroll
         for (int iv = 0; iv < NV; iv++) soa_st(S.out, iv * G.ncell, cb, w[iv]);
+      if (q < COMB_MAXST - 1)
 #pragma unroll
-      for (int iv = 0; iv < NV; iv++) {
-        if (q == 0) r0[iv] = w[iv];
-        if (q == 1) r1[iv] = w[iv];
-      }
+        for (int iv = 0; iv < NV; iv++) r[q][iv] = w[iv];
     }
   }
   KD_INL void operator()(long i) const
@@ -933,7 +938,7 @@ struct OpCombSeq {
     if (i < nzero) zero[i] = 0u;
     const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
     const long c = cidx(G, ix, iy, iz);
-    double coef[3][3];
+    double coef[COMB_MAXST][3];
     coefs(coef);
     for (int iv = 0; iv < nv; iv++) var((long)iv * G.ncell + c, coef);
   }
@@ -947,7 +952,7 @@ struct OpCombSeqN : OpCombSeq {
   {
     if (i < nzero) zero[i] = 0u;
     const int ix = (int)(i % G.nx), iy = (int)((i / G.nx) % G.ny), iz = (int)(i / ((long)G.nx * G.ny));
-    double coef[3][3];
+    double coef[COMB_MAXST][3];
     coefs(coef);
     this->template cell_vars<NV>(cidx(G, ix, iy, iz), coef);
   }
@@ -1694,8 +1699,7 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   // each seq() precedes a u2p and also clears the four list counters of the
   // u2p and of the explicit operator that follows it (no reader of them runs
   // in between: the floor passes of the previous sweeps were joined)
-  // the statements of one launch; with `launch` false the op is returned
-  // for the update to evaluate (the counters are then cleared by it)
+  // the statements of one launch (at most COMB_MAXST)
   auto make = [&](std::initializer_list<CombStmt> l) {
     OpCombSeq op{s.G, s.P.nv, (int)l.size(), {}, s.clock, gamma};
     int q = 0;
@@ -1716,12 +1720,14 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
     explicit_op(ex, s, !polar_correction_active(s), true);
     seq(l);
   };
-  if (!s.P.rad) {
-    // no implicit source between them (implicit_op is empty): ut0 = u;
-    // drt1 = (u - ut0) / (gamma dt) with ut0 as just computed; ut1 = u
-    seq({CombStmt{s.ut0, 1, {A(one, s.u), none, none}},
-         CombStmt{s.drt1, 2, {A(idtg, s.u), R(midtg, 0), none}},
-         CombStmt{s.ut1, 1, {A(one, s.u), none, none}}});
+  // Without radiation implicit_op is empty, so u does not change between
+  // the first copies: ut1 = u = ut0 (1 u, the same bits), and drt1 =
+  // (u - ut0) / (gamma dt) = idtg ut0 + midtg ut0, which the launches that
+  // read drt1 recompute from ut0 with the same operations instead of
+  // storing it.  Neither ut1 (first stage) nor drt1 is then stored or read.
+  const bool norad = !s.P.rad;
+  if (norad) {
+    seq({CombStmt{s.ut0, 1, {A(one, s.u), none, none}}});
   } else {
     comb(ex, s, gamma, s.ut0, one, s.u);
     implicit_op(ex, s, gamma);
@@ -1731,9 +1737,15 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   u2p(ex, s, true);
   do_correct(ex, s);
   // explicit; dut1 = (u - ut1) / dt; u = ut0 + dt dut1 + (1 - 2 gamma) dt drt1; uforget = u
-  explicit_then({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
-                 CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
-                 CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
+  if (norad)
+    explicit_then({CombStmt{nullptr, 2, {A(idtg, s.ut0), A(midtg, s.ut0), none}},
+                   CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut0), none}},
+                   CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 1), R(dtc, 0)}},
+                   CombStmt{s.uforget, 1, {R(one, 2), none, none}}});
+  else
+    explicit_then({CombStmt{s.dut1, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+                   CombStmt{s.u, 3, {A(one, s.ut0), R(dt1, 0), A(dtc, s.drt1)}},
+                   CombStmt{s.uforget, 1, {R(one, 1), none, none}}});
   u2p(ex, s, true);
   do_correct(ex, s);
   implicit_op(ex, s, gamma);
@@ -1743,9 +1755,15 @@ void step_rk2imex(Ex &ex, const GridBufs &s)
   u2p(ex, s, true);
   do_correct(ex, s);
   // explicit; dut2 = (u - ut2) / dt; u = ut0 + dt/2 (dut1 + dut2); u = u + dt/2 (drt1 + drt2)
-  explicit_then({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
-                 CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
-                 CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
+  if (norad)
+    explicit_then({CombStmt{nullptr, 2, {A(idtg, s.ut0), A(midtg, s.ut0), none}},
+                   CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+                   CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 1)}},
+                   CombStmt{s.u, 3, {R(one, 2), R(dth, 0), A(dth, s.drt2)}}});
+  else
+    explicit_then({CombStmt{s.dut2, 2, {A(idt, s.u), A(midt, s.ut1), none}},
+                   CombStmt{nullptr, 3, {A(one, s.ut0), A(dth, s.dut1), R(dth, 0)}},
+                   CombStmt{s.u, 3, {R(one, 1), A(dth, s.drt1), A(dth, s.drt2)}}});
   u2p(ex, s, true);
   do_correct(ex, s);
 }
