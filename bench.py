@@ -36,10 +36,10 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # The face kernels per sweep direction (x1, x2, x3) and schedule (None: not
 # measured); the update per variant (tune.update_pre).
 GRID_TRAFFIC_128x128x64 = {
-    "u2p": 314e6,
-    "update": {1: 587e6},
-    "flux": {"split": (469e6, None, 307e6), "fused": (None, 791e6, None)},
-    "recon": {"split": (245e6, None, 357e6)},
+    "u2p": 316e6,
+    "update": {1: 594e6},
+    "flux": {"split": (474e6, None, 308e6), "fused": (None, 794e6, None)},
+    "recon": {"split": (246e6, None, 358e6)},
 }
 
 # FP64 operations per zone-update of the GRMHD RK2IMEX step on the
@@ -439,6 +439,10 @@ def bench_grid(args, kind):
     elif not args.no_autotune:
         t0 = time.perf_counter()
         rows = dg.autotune(steps=args.tune_steps)
+        if args.tune_log:
+            with open(args.tune_log, "w") as f:
+                for name, val, ms in rows:
+                    f.write("%-28s %8s %8.4f\n" % (name, "" if val is None else val, ms))
         tn = dg.get_tune()
         names = {k: dg.L.koral_hip_grid_op_name(k).decode() for k in range(32)}
         chosen = [r for r in rows if r[0].startswith("chose")]
@@ -630,6 +634,7 @@ def main():
                     help="launch choices to use instead of the search, as the line's config.autotune reports "
                          "them (fused, floor_wave, update_pre, comb_nv, floor_nw, u2p_nw, block)")
     ap.add_argument("--tune-steps", type=int, default=3)
+    ap.add_argument("--tune-log", default=None, help="write the autotuner's search (choice, value, ms per step) here")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the single-GPU build holds the whole domain "
