@@ -963,12 +963,13 @@ struct OpCombSeqN : OpCombSeq {
 // ---------------------------------------------------------------------------
 // PRE: the cell's own u and p are loaded together with the fluxes, so their
 // loads are in flight at once instead of in three dependent rounds (more
-// registers: 3 waves per SIMD); otherwise p is loaded for the metric source
-// and u at the end.  The arithmetic of every term is the same either way.
+// registers: 128 VGPRs for MHD, so still 4 waves per SIMD; 3 with
+// radiation); otherwise p is loaded for the metric source and u at the end.
+// The arithmetic of every term is the same either way.
 template <int MAG, int RAD, bool PRE = false>
 struct OpUpdate {
   static constexpr int kind = K_UPDATE;
-  static constexpr int waves_per_simd = (RAD || PRE) ? 3 : 4;
+  static constexpr int waves_per_simd = RAD ? 3 : 4;
   koral_params P;
   DGrid G;
   const double *p;
