@@ -548,7 +548,9 @@ KD_INL void load_face_states(const double *__restrict__ fs, long nb, long i, dou
 template <int MAG, int RAD, int D>
 struct OpFlux {
   static constexpr int kind = K_FLUX;
-  static constexpr int waves_per_simd = 2;
+  // 3 waves per SIMD for MHD (168 VGPRs, 10-18 spilled): 2.5 % faster steps
+  // than 2 waves with no spill
+  static constexpr int waves_per_simd = RAD ? 2 : 3;
   koral_params P;
   DGrid G;
   const double *fs, *ws;
@@ -642,7 +644,7 @@ KD_INL void face_states(const koral_params &P, const DGrid &G, const double *__r
 template <int MAG, int RAD, int ORDER, int D>
 struct OpSweep {
   static constexpr int kind = K_FLUX;
-  static constexpr int waves_per_simd = 2;
+  static constexpr int waves_per_simd = RAD ? 2 : 3;  // as OpFlux
   koral_params P;
   DGrid G;
   const double *p, *ws;
