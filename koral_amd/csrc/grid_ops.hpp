@@ -126,27 +126,40 @@ struct OpBcFaces {
     const Box b{0, 0, 0, d == 0 ? 2 * ng : G.nx, d == 1 ? 2 * ng : G.ny, d == 2 ? 2 * ng : G.nz};
     int ii[3];
     b.at(i, ii[0], ii[1], ii[2]);
-    const int hi = ii[d] >= ng;
-    ii[d] = hi ? n + ii[d] - ng : ii[d] - ng;
-    int src[3] = {ii[0], ii[1], ii[2]};
-    int type = G.bc[2 * d + hi];
+    // direction d through constant indices (k == d), so no array here is
+    // indexed at run time and none lives in scratch memory
+    int src[3];
     bool flip = false;
-    if (type == BC_PERIODIC) {
-      if (src[d] < 0) src[d] += n;
-      if (src[d] > n - 1) src[d] -= n;
-    } else if (type == BC_REFLECT) {
-      src[d] = hi ? 2 * n - src[d] - 1 : -src[d] - 1;
-      flip = true;
-    } else {
-      src[d] = hi ? n - 1 : 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (k == d) {
+        const int hi = ii[k] >= ng;
+        ii[k] = hi ? n + ii[k] - ng : ii[k] - ng;
+        int sk = ii[k];
+        const int type = G.bc[2 * k + hi];
+        if (type == BC_PERIODIC) {
+          if (sk < 0) sk += n;
+          if (sk > n - 1) sk -= n;
+        } else if (type == BC_REFLECT) {
+          sk = hi ? 2 * n - sk - 1 : -sk - 1;
+          flip = true;
+        } else {
+          sk = hi ? n - 1 : 0;
+        }
+        src[k] = sk;
+      } else {
+        src[k] = ii[k];
+      }
     }
     double pv[NV];
     load_p<MAG, RAD>(p, G.ncell, cidx(G, src[0], src[1], src[2]), pv);
-    if (flip) {
-      pv[2 + d] = -pv[2 + d];
-      if (MAG) pv[6 + d] = -pv[6 + d];
-      if (RAD) pv[Lay<MAG, RAD>::EE + 1 + d] = -pv[Lay<MAG, RAD>::EE + 1 + d];
-    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (flip && k == d) {
+        pv[2 + k] = -pv[2 + k];
+        if (MAG) pv[6 + k] = -pv[6 + k];
+        if (RAD) pv[Lay<MAG, RAD>::EE + 1 + k] = -pv[Lay<MAG, RAD>::EE + 1 + k];
+      }
     long c = cidx(G, ii[0], ii[1], ii[2]);
     store_p<MAG, RAD>(p, G.ncell, c, pv);
     Geom g;
