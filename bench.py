@@ -36,10 +36,10 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # The face kernels per sweep direction (x1, x2, x3) and schedule (None: not
 # measured); the update per variant (tune.update_pre).
 GRID_TRAFFIC_128x128x64 = {
-    "u2p": 316e6,
-    "update": {1: 594e6},
-    "flux": {"split": (474e6, None, 308e6), "fused": (None, 794e6, None)},
-    "recon": {"split": (246e6, None, 358e6)},
+    "u2p": 315e6,
+    "update": {1: 581e6},
+    "flux": {"split": (None, None, None), "fused": (448e6, 880e6, 398e6)},
+    "recon": {"split": (None, None, None)},
 }
 
 # FP64 operations per zone-update of the GRMHD RK2IMEX step on the
@@ -57,6 +57,13 @@ GRMHD_FLOPS_PER_ZONE = {"sweeps": 16581.7, "u2p": 5243.4, "update": 1356.0, "wav
 # once per (x1, x2) plane as this layout stores it (amortised over NZ = 64,
 # about 760 B), and counted per cell in 3-D as KORAL tabulates it (4288 B).
 GRMHD_BYTES_PER_ZONE = {"axisymmetric_metric": 760.0, "metric_3d": 4288.0}
+# The same count per operator and launch: the x1, x2, x3 sweeps (ghost
+# fill excluded; reconstruction, floors, fluxes and HLL of one stage), the
+# update, u2p and wave speeds, in FP64 operations per zone of one launch
+# (per zone-update divided by the launches of a step: 2 explicit stages, 4
+# u2p passes).
+GRMHD_FLOPS_PER_LAUNCH_ZONE = {"sweep": (5505.1 / 2, 5517.8 / 2, 5558.8 / 2), "update": 1356.0 / 2,
+                               "u2p": 5243.4 / 4, "wavespeeds": 1240.6 / 2}
 
 # Algorithmic FP64 operations per implicit solve of the seeded batch workload
 # (all six fallback attempts included), counted by the operation-counting
@@ -505,8 +512,21 @@ def bench_grid(args, kind):
             traffic = sum(v) / len(v) if None not in v else None
         elif dom == "update":
             traffic = tr.get(tn.update_pre)
+        elif dom == "recon":
+            v = [tr["split"][d] for d in dims if not tn.fused[d]]
+            traffic = sum(v) / len(v) if v and None not in v else None
         elif isinstance(tr, float):
             traffic = tr
+    # FP64 rate of the dominant kernel from the operation count (the face
+    # kernels only when every sweep is fused: a split sweep's flux kernel
+    # does not do the reconstruction the count includes)
+    fp64 = None
+    if kind == "grmhd":
+        fz = GRMHD_FLOPS_PER_LAUNCH_ZONE
+        if dom == "flux" and nfused == len(dims):
+            fp64 = sum(fz["sweep"][d] for d in dims) / len(dims) * zones
+        elif dom in fz and dom != "sweep":
+            fp64 = fz[dom] * zones
     step_s = wall / args.steps
     zups = zones * args.steps / wall
     step_info = None
@@ -524,6 +544,27 @@ def bench_grid(args, kind):
             step_info["sweeps_ms_per_step"] = sw_ms
             step_info["sweeps_fp64_TFLOPs"] = sw
             step_info["sweeps_fp64_frac"] = sw / FP64_PEAK_TFLOPS
+    # the roof the dominant kernel is closer to: HBM by this layout's
+    # compulsory bytes, or FP64 by the operation count; the other alongside
+    hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "algorithmic_bytes_per_launch": bytes_launch,
+           "algorithmic_bytes_basis": "compulsory bytes of this layout (metric in per-(x1,x2) tables, "
+                                      "served by the caches)"}
+    comp = None
+    if fp64:
+        fa = fp64 / per_launch / 1e12
+        comp = {"achieved": fa, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fa / FP64_PEAK_TFLOPS,
+                "algorithmic_flops_per_launch": fp64,
+                "compute": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)"}
+    if comp and comp["frac"] > hbm["frac"]:
+        roof = dict(bound="mfma", **comp)
+        roof["hbm"] = hbm
+    else:
+        roof = dict(bound="hbm", **hbm)
+        roof["fp64"] = comp
+    roof.update({"traffic": traffic, "traffic_GBps": traffic / per_launch / 1e9 if traffic else None,
+                 "kernel": dom, "kernel_ms": per_launch * 1e3, "launches": nl, "fused_sweeps": nfused,
+                 "step": step_info})
     res = {
         "metric": METRIC,
         "value": zups,
@@ -543,14 +584,7 @@ def bench_grid(args, kind):
                    if kind == "grmhd" else None,
                    "counters_per_step": (counters / args.steps).tolist(),
                    "operator_ms_per_step_pilot": {k: v[0] / 3 for k, v in sorted(split.items(), key=lambda kv: -kv[1][0])}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_GBps": traffic / per_launch / 1e9 if traffic else None,
-                     "kernel": dom, "kernel_ms": per_launch * 1e3, "launches": nl,
-                     "fused_sweeps": nfused, "algorithmic_bytes_per_launch": bytes_launch,
-                     "algorithmic_bytes_basis": "compulsory bytes of this layout (metric in per-(x1,x2) tables, "
-                                                "served by the caches)",
-                     "step": step_info},
+        "roofline": roof,
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_grid(P, kind, args, n)
