@@ -342,9 +342,19 @@ typedef struct koral_grid_tune {
   int update_pre;   /* update kernel: 1 loads the cell's u and p with the fluxes */
   int comb_nv;        /* 1: the stage-algebra kernel is compiled for the
                          grid's variable count */
+  int u2p_budget;     /* residual evaluations the common u2p pass spends on
+                         a cell's hot inversion before leaving the cell to
+                         the deferred (wave-parallel) pass; 0: no bound */
   long floor_nw, u2p_nw;
 } koral_grid_tune;
 int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t);
+/* Lengths of the deferred work lists (no reference counterpart; a
+   diagnostic of the launch choices): counts[0..2] the faces of the x1, x2,
+   x3 sweeps of the last koral_hip_op_explicit left to the b^2/rho floor
+   pass, counts[3] the cells of the last u2p left to the full
+   calc_primitives pass (also after a step; a step reuses the face counts).
+   Synchronises the grid's stream. */
+int koral_hip_grid_deferred_counts(koral_hip_grid *g, unsigned counts[4]);
 int koral_hip_grid_set_tune(koral_hip_grid *g, const koral_grid_tune *t);
 /* Coordinate search over the choices above on the grid's current state at
    time t with step dt: each candidate runs 1 + steps RK2IMEX steps from a

@@ -308,7 +308,10 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 
 // Wstart > 0 replaces the W start computed from pp (the learned W guess,
 // which has no reference counterpart); iters, when given, receives the
-// Newton iterations made.
+// Newton iterations made.  budget > 0 (sequential path only) bounds the
+// residual evaluations: the solver gives up with U2P_OVER_BUDGET, pp
+// untouched, when a point needs more (a caller that can hand the point to
+// the wave-parallel solver redoes it there from the start, same result).
 //
 // WAVE: all 64 lanes of a wave run the inversion of one point (uniform
 // inputs).  The two trial loops of the iteration -- W *= 10 until the
@@ -327,9 +330,10 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 #else
 #define KD_WAVE_SOLVER 1
 #endif
+constexpr int U2P_OVER_BUDGET = -200;
 template <int MAG, int RAD, bool WAVE = false>
 KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype,
-                        double Wstart = 0., int *iters = nullptr)
+                        double Wstart = 0., int *iters = nullptr, int budget = 0)
 {
   double gdetu = g.gdet;
   double alpha = sqrt(-1. / g.G[0][0]);
@@ -377,6 +381,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   double W = Wstart > 0. ? Wstart : (pp[0] + P.gamma * pp[1]) * gamma2;
   double f0, dfdW, err;
   int i_increase = 0;
+  int nev = 0;  // residual evaluations of the sequential path (budget)
   if (iters) *iters = 0;
 #if KD_WAVE_SOLVER
   if constexpr (WAVE) {
@@ -397,6 +402,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   {
     for (;;) {
       f0 = dfdW = 0.;
+      if (budget > 0 && ++nev > budget) return U2P_OVER_BUDGET;
       f_u2p(P, etype, W - D, c, f0, dfdW, err);
       if ((w_bad(W, Bsq, QdotBsq, Qtsq) || !isfinite(f0) || !isfinite(dfdW)) && i_increase < 50) {
         W *= 10.;
@@ -424,6 +430,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
       dfdW = dfc;
       err = ec;
     } else {
+      if (budget > 0 && ++nev > budget) return U2P_OVER_BUDGET;
       f_u2p(P, etype, W - D, c, f0, dfdW, err);
     }
     have = false;
@@ -468,6 +475,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
       double dumpfac = 1.;
       for (;;) {
         double f0t = 0., dft = 0., errt;
+        if (budget > 0 && ++nev > budget) return U2P_OVER_BUDGET;
         f_u2p(P, etype, Wnew - D, c, f0t, dft, errt);
         if ((w_bad(Wnew, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft)) && idump < 100) {
           idump++;
@@ -711,10 +719,12 @@ KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 // path.  A failed hot inversion leaves pp as it was unless it failed with
 // -105 (negative density after the update), so such a redo may start at the
 // entropy inversion (SKIP_HOT) and reach the same state; U2P_DEFERRED_105
-// marks the other case.
+// marks the other cases, where the redo runs the hot inversion again: -105,
+// and a hot inversion stopped by the budget (DEFER only, see u2p_solver_W).
 constexpr int U2P_DEFERRED = 1, U2P_DEFERRED_105 = 2;
 template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
-KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups)
+KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups,
+               int budget = 0)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   double uu[NV], ppbak[NV];
@@ -733,9 +743,9 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
     uu[0] = P.rhofloor * gdetu;
     ret = -2;
   }
-  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0);
+  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0, 0., nullptr, DEFER ? budget : 0);
   if (u2pret < 0) {
-    if (DEFER) return u2pret == -105 ? U2P_DEFERRED_105 : U2P_DEFERRED;
+    if (DEFER) return (u2pret == -105 || u2pret == U2P_OVER_BUDGET) ? U2P_DEFERRED_105 : U2P_DEFERRED;
     ret = -1;
     u2pret = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 1);
     if (u2pret < 0) ret = -2;
@@ -874,12 +884,12 @@ KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 constexpr int CP_DEFERRED = 100, CP_DEFERRED_HOT = 101;
 template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
 KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
-                               int &fl_entr2, int &fl_hd, int &fl_rad)
+                               int &fl_entr2, int &fl_hd, int &fl_rad, int budget = 0)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};
   fl_entr = 0;
   fl_entr2 = 0;
-  int r = u2p<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, corrected, fixups);
+  int r = u2p<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, corrected, fixups, budget);
   if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED_HOT;
   if (DEFER && r == U2P_DEFERRED_105) return CP_DEFERRED;
   if (corrected[0] == 1) fl_entr = 1;

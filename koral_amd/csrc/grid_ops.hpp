@@ -1057,6 +1057,10 @@ struct OpU2P {
   unsigned *list, *cnt;
   KD_HD long size() const { return (long)G.nx * G.ny * G.nz; }
   unsigned char *why;  // per list entry: 1 when the redo may skip the failed hot inversion
+  // residual evaluations the hot inversion of the common pass may spend
+  // before the cell is left to the deferred pass (0: no bound); a cell with
+  // a long run of W bumps or step halvings then runs them lane-parallel
+  int budget = 0;
   KD_INL Cnt3 operator()(long i) const { return cell<true, false, false>(i, list, cnt, true); }
   // the deferring calc_primitives, or the full one (optionally from the
   // entropy inversion on) on cell i; WAVE: the calling wave's lanes all run
@@ -1075,7 +1079,7 @@ struct OpU2P {
     Geom g;
     geom_cell(G, ix, iy, g);
     int fe, fe2, fh, fr;
-    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, fe, fe2, fh, fr);
+    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, fe, fe2, fh, fr, DEFER ? budget : 0);
     if (DEFER) {
       const bool defer = r == CP_DEFERRED || r == CP_DEFERRED_HOT;
       const long slot = append_index(defer, (unsigned)i, lst, len);
@@ -1280,6 +1284,7 @@ struct GridTune {
   int floor_wave;       // deferred passes: 1 one wave per face or cell, 0 one lane
   int update_pre;       // update: 1 the cell's u and p loaded with the fluxes
   int comb_nv;          // 1: stage algebra with the variable count a template parameter
+  int u2p_budget;       // OpU2P::budget
   long floor_nw;        // waves (or lanes) in flight in the deferred face passes
   long u2p_nw;          // waves (or lanes) in flight in the deferred u2p pass
 };
@@ -1291,7 +1296,8 @@ inline GridTune default_tune()
   t.floor_wave = 1;
   t.update_pre = 0;
   t.comb_nv = 1;
-  t.floor_nw = 1024;
+  t.u2p_budget = 32;
+  t.floor_nw = 4096;
   t.u2p_nw = 4096;
   return t;
 }
@@ -1579,7 +1585,7 @@ void u2p(Ex &ex, const GridBufs &s, bool count_cleared = false)
     using T = decltype(t);
     if (!count_cleared) ex.each(OpZeroU32{s.fcnt + 3, 1});
     KD_CHECK(!count_cleared || s.fcnt[3] == 0u, "u2p: cell-list count not cleared", s.fcnt[3], 0, 0);
-    const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy};
+    const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy, s.tune.u2p_budget};
     ex.count(op, s.counters + 3);
     if (s.tune.floor_wave)
       ex.count(OpU2PFloor<T::MAG, T::RAD, true>{op, s.tune.u2p_nw}, s.counters + 3);

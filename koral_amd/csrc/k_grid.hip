@@ -870,6 +870,7 @@ void tune_to_c(const GridTune &a, koral_grid_tune *t)
   t->floor_wave = a.floor_wave;
   t->update_pre = a.update_pre;
   t->comb_nv = a.comb_nv;
+  t->u2p_budget = a.u2p_budget;
   t->floor_nw = a.floor_nw;
   t->u2p_nw = a.u2p_nw;
 }
@@ -879,6 +880,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
   for (int k = 0; k < TUNE_KINDS; k++)
     if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
       return set_error(KORAL_E_ARG, "set_tune: block sizes must be 64, 128 or 256");
+  if (t->u2p_budget < 0) return set_error(KORAL_E_ARG, "set_tune: u2p_budget must be >= 0");
   if (t->floor_nw < 1 || t->floor_nw > (1L << 20) || t->u2p_nw < 1 || t->u2p_nw > (1L << 20))
     return set_error(KORAL_E_ARG, "set_tune: waves in flight must be in [1, 2^20]");
   for (int k = 0; k < TUNE_KINDS; k++) a.block[k] = t->block[k];
@@ -886,6 +888,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
   a.floor_wave = t->floor_wave ? 1 : 0;
   a.update_pre = t->update_pre ? 1 : 0;
   a.comb_nv = t->comb_nv ? 1 : 0;
+  a.u2p_budget = t->u2p_budget;
   a.floor_nw = t->floor_nw;
   a.u2p_nw = t->u2p_nw;
   return 0;
@@ -949,6 +952,15 @@ int tune_trial(koral_hip_grid *g, TuneSnap &snap, int steps, hipEvent_t e0, hipE
 }  // namespace
 
 extern "C" {
+
+int koral_hip_grid_deferred_counts(koral_hip_grid *g, unsigned counts[4])
+{
+  if (!g || !counts) return set_error(KORAL_E_ARG, "deferred_counts: bad arguments");
+  int rc = check_hip(hipMemcpyAsync(counts, g->b.fcnt, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, g->s),
+                     "deferred_counts copy");
+  if (!rc) rc = check_hip(hipStreamSynchronize(g->s), "deferred_counts sync");
+  return rc;
+}
 
 int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t)
 {
@@ -1024,11 +1036,12 @@ int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, c
       coord(fname[d], [&] { return (long)T.fused[d]; }, [&](long v) { T.fused[d] = (int)v; }, {0, 1});
   if (g->b.P.mag) {
     coord("floor_wave", [&] { return (long)T.floor_wave; }, [&](long v) { T.floor_wave = (int)v; }, {1, 0});
-    coord("floor_nw", [&] { return T.floor_nw; }, [&](long v) { T.floor_nw = v; }, {256, 1024, 4096});
+    coord("floor_nw", [&] { return T.floor_nw; }, [&](long v) { T.floor_nw = v; }, {1024, 4096, 16384});
   }
   coord("u2p_nw", [&] { return T.u2p_nw; }, [&](long v) { T.u2p_nw = v; }, {1024, 4096, 16384});
   coord("update_pre", [&] { return (long)T.update_pre; }, [&](long v) { T.update_pre = (int)v; }, {0, 1});
   coord("comb_nv", [&] { return (long)T.comb_nv; }, [&](long v) { T.comb_nv = (int)v; }, {1, 0});
+  coord("u2p_budget", [&] { return (long)T.u2p_budget; }, [&](long v) { T.u2p_budget = (int)v; }, {32, 0, 16, 64});
   // block sizes of the operator kinds a step launches
   const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
                        K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};

@@ -19,7 +19,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class GridTune(ctypes.Structure):
     """koral_grid_tune: the launch choices of a grid (include/koral_hip.h)."""
     _fields_ = [("block", ctypes.c_int * 32), ("fused", ctypes.c_int * 3), ("floor_wave", ctypes.c_int),
-                ("update_pre", ctypes.c_int), ("comb_nv", ctypes.c_int),
+                ("update_pre", ctypes.c_int), ("comb_nv", ctypes.c_int), ("u2p_budget", ctypes.c_int),
                 ("floor_nw", ctypes.c_long),
                 ("u2p_nw", ctypes.c_long)]
 
@@ -58,6 +58,7 @@ def _sig(L):
         "koral_hip_grid_write_restart": [_vp, ctypes.c_char_p, i, i, d, i],
         "koral_hip_grid_read_restart": [_vp, ctypes.c_char_p, i, _vp, _vp, _vp],
         "koral_hip_grid_get_tune": [_vp, _vp],
+        "koral_hip_grid_deferred_counts": [_vp, _vp],
         "koral_hip_grid_set_tune": [_vp, _vp],
         "koral_hip_grid_autotune": [_vp, d, d, i, ctypes.c_char_p, i],
     }.items():
@@ -225,6 +226,13 @@ class DeviceGrid:
         check(r, "read_restart")
         self.t = t.value
         return True, n1.value
+
+    def deferred_counts(self):
+        """[x1, x2, x3 floor faces of the last explicit operator, cells left
+        to the full calc_primitives by the last u2p]."""
+        c = np.zeros(4, np.uint32)
+        check(self.L.koral_hip_grid_deferred_counts(self.h, c.ctypes.data_as(_vp)), "deferred_counts")
+        return c.tolist()
 
     def get_tune(self):
         t = GridTune()

@@ -454,13 +454,15 @@ int main(int argc, char **argv)
   // "fused" (reconstruction inside the flux functor for every direction),
   // "lane" (one lane per face or cell in the deferred passes), "pre" (the
   // update loading u and p with the fluxes), "combrt" (the stage algebra
-  // with a run-time variable count), or several
+  // with a run-time variable count), "budget" (a u2p budget of 3 residual
+  // evaluations, so most cells take the deferred pass), or several
   if (const char *tn = getenv("KORAL_HC_TUNE")) {
     if (strstr(tn, "fused"))
       for (int d = 0; d < 3; d++) s.tune.fused[d] = 1;
     if (strstr(tn, "lane")) s.tune.floor_wave = 0;
     if (strstr(tn, "pre")) s.tune.update_pre = 1;
     if (strstr(tn, "combrt")) s.tune.comb_nv = 0;
+    if (strstr(tn, "budget")) s.tune.u2p_budget = 3;
   }
   s.P = P;
   DGrid &G = s.G;

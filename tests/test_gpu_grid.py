@@ -350,6 +350,7 @@ def test_grid_launch_choices_bit_identical(kind):
         t.floor_wave = wave
         t.update_pre = fused
         t.comb_nv = wave
+        t.u2p_budget = 3 if fused else 0
         for k in range(32):
             t.block[k] = block
         t.floor_nw, t.u2p_nw = floor_nw, u2p_nw

@@ -175,7 +175,7 @@ def test_host_full_size_grrmhd_m1(tmp_path):
 
 
 @pytest.mark.parametrize("kind", ["tube", "torus_mhd", "torus_rad"])
-@pytest.mark.parametrize("tune", ["fused", "fused,lane,pre,combrt", "lane", "pre"])
+@pytest.mark.parametrize("tune", ["fused", "fused,lane,pre,combrt", "lane,budget", "pre,budget"])
 def test_host_step_launch_choices_match_oracle(oracle, exe, tmp_path, kind, tune):
     """The launch choices the autotuner picks between run different functors
     (reconstruction inside the flux functor, one lane instead of one wave per
