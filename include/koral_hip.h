@@ -333,7 +333,7 @@ int koral_hip_grid_use_graph(koral_hip_grid *g, int enable);
    deferred passes (the few faces needing the b^2/rho floor re-inversion and
    the cells needing the entropy inversion or that floor): one wave per face
    or cell (floor_wave = 1, the inversion's trials one per lane) or one lane
-   per face or cell (0), with floor_nw / u2p_nw waves (or lanes) in flight.
+   per face or cell (0), with floor_nw / u2p_nw faces or cells in flight.
    Every choice computes the same results bit for bit. */
 typedef struct koral_grid_tune {
   int block[32];

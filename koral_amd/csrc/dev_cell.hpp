@@ -313,23 +313,29 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 // untouched, when a point needs more (a caller that can hand the point to
 // the wave-parallel solver redoes it there from the start, same result).
 //
-// WAVE: all 64 lanes of a wave run the inversion of one point (uniform
-// inputs).  The two trial loops of the iteration -- W *= 10 until the
-// residual is admissible (at most 50 times), and the step halving until the
-// trial W is admissible (at most 100 halvings) -- are searches for the
-// first admissible trial; the lanes evaluate one trial each (trial k is
+// WAVE: a group of WAVE_GROUP lanes (an aligned part of a wave) runs the
+// inversion of one point (uniform inputs within the group).  The two trial
+// loops of the iteration -- W *= 10 until the residual is admissible (trials
+// 0..49), and the step halving until the trial W is admissible (trials
+// 0..99) -- are searches for the first admissible trial; the lanes of the
+// group evaluate one trial each, in rounds of WAVE_GROUP trials (trial k is
 // formed with the same k sequential multiplications, or as 2^-k, which k
-// halvings of 1 give exactly) and take
-// the first admissible one by ballot, which is where the sequential loop
-// stops, so the result is the same bit for bit.  A point whose iteration
-// halves its step many times then costs a few evaluations of the residual
-// instead of up to a hundred in sequence; the residual at the accepted trial
-// also serves as the next iteration's residual.
+// halvings of 1 give exactly), and take the first admissible one by a
+// ballot restricted to the group, which is where the sequential loop stops,
+// so the result is the same bit for bit.  A point whose iteration halves its
+// step many times then costs a few evaluations of the residual instead of
+// up to a hundred in sequence; the residual at the accepted trial also
+// serves as the next iteration's residual.  Groups of 16 lanes (four points
+// per wave) measured slower than whole waves on a torus whose deferred faces
+// need long runs of step halvings (5.9 against 4.6 ms per step), so a group
+// is the whole wave.
 #ifdef KD_HOST_CHECK
 #define KD_WAVE_SOLVER 0
 #else
 #define KD_WAVE_SOLVER 1
 #endif
+constexpr int WAVE_GROUP = 64;  // lanes per point of the WAVE solver (divides 64)
+static_assert(64 % WAVE_GROUP == 0, "WAVE_GROUP divides the wave");
 constexpr int U2P_OVER_BUDGET = -200;
 template <int MAG, int RAD, bool WAVE = false>
 KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype,
@@ -385,18 +391,26 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   if (iters) *iters = 0;
 #if KD_WAVE_SOLVER
   if constexpr (WAVE) {
-    const int lane = threadIdx.x & 63;
-    double Wk = W;
-    for (int k = 0; k < lane; k++) Wk *= 10.;
-    bool ok = false;
-    if (lane < 50) {
-      double f = 0., df = 0., e;
-      f_u2p(P, etype, Wk - D, c, f, df, e);
-      ok = !(w_bad(Wk, Bsq, QdotBsq, Qtsq) || !isfinite(f) || !isfinite(df));
+    const int lane = threadIdx.x & 63, sub = lane % WAVE_GROUP, g0 = lane - sub;
+    const unsigned long long gmask = (WAVE_GROUP == 64) ? ~0ull : (((1ull << WAVE_GROUP) - 1ull) << g0);
+    bool found = false;
+    for (int r = 0; r * WAVE_GROUP < 50 && !found; r++) {
+      const int k = r * WAVE_GROUP + sub;
+      double Wk = W;
+      for (int q = 0; q < k; q++) Wk *= 10.;
+      bool ok = false;
+      if (k < 50) {
+        double f = 0., df = 0., e;
+        f_u2p(P, etype, Wk - D, c, f, df, e);
+        ok = !(w_bad(Wk, Bsq, QdotBsq, Qtsq) || !isfinite(f) || !isfinite(df));
+      }
+      const unsigned long long m = __ballot(ok) & gmask;
+      if (m) {
+        W = __shfl(Wk, __ffsll(m) - 1);
+        found = true;
+      }
     }
-    const unsigned long long m = __ballot(ok);
-    if (!m) return -150;
-    W = __shfl(Wk, __ffsll(m) - 1);
+    if (!found) return -150;
   } else
 #endif
   {
@@ -442,11 +456,12 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
     double Wnew = W - f0 / dfdW;
 #if KD_WAVE_SOLVER
     if constexpr (WAVE) {
-      // trials k = 0..99 in two rounds of 64; trial k halves the step k times
-      const int lane = threadIdx.x & 63;
+      // trials k = 0..99 in rounds of WAVE_GROUP; trial k halves the step k times
+      const int lane = threadIdx.x & 63, sub = lane % WAVE_GROUP, g0 = lane - sub;
+      const unsigned long long gmask = (WAVE_GROUP == 64) ? ~0ull : (((1ull << WAVE_GROUP) - 1ull) << g0);
       int found = -1;
-      for (int round = 0; round < 2 && found < 0; round++) {
-        const int k = round * 64 + lane;
+      for (int round = 0; round * WAVE_GROUP < 100 && found < 0; round++) {
+        const int k = round * WAVE_GROUP + sub;
         // k halvings of 1 are exact: 2^-k
         const double dumpfac = ldexp(1., -k);
         const double Wt = k == 0 ? Wnew : W - dumpfac * f0 / dfdW;
@@ -456,10 +471,10 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
           f_u2p(P, etype, Wt - D, c, f0t, dft, errt);
           ok = !(w_bad(Wt, Bsq, QdotBsq, Qtsq) || !isfinite(f0t) || !isfinite(dft));
         }
-        const unsigned long long m = __ballot(ok);
+        const unsigned long long m = __ballot(ok) & gmask;
         if (m) {
           const int win = __ffsll(m) - 1;
-          found = round * 64 + win;
+          found = round * WAVE_GROUP + win - g0;
           Wnew = __shfl(Wt, win);
           fc = __shfl(f0t, win);
           dfc = __shfl(dft, win);

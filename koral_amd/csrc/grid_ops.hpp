@@ -590,10 +590,10 @@ struct OpFlux {
 };
 
 // The faces OpFlux left: the same face states, the full face floors, the
-// same fluxes.  WAVE: one wave per face (the floors' W inversions evaluate
-// their trials one per lane; lane 0 stores); otherwise one lane per face.
-// nw waves (WAVE) or lanes stride over the list, whose length is known only
-// on the device.
+// same fluxes.  WAVE: a group of WAVE_GROUP lanes per face (the floors' W
+// inversions evaluate their trials one per lane; the group's first lane
+// stores); otherwise one lane per face.  nw groups (WAVE) or lanes stride
+// over the list, whose length is known only on the device.
 template <int MAG, int RAD, int D, bool WAVE = true>
 struct OpFluxFloor {
   static constexpr int kind = K_FLUX_FLOOR;
@@ -605,13 +605,13 @@ struct OpFluxFloor {
   const unsigned *list, *cnt;
   Box b;
   long nw;
-  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
+  KD_HD long size() const { return WAVE ? (long)WAVE_GROUP * nw : nw; }
   KD_INL void operator()(long t) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     const long m = (long)*cnt;
-    const bool lead = WAVE ? (t & 63) == 0 : true;
-    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
+    const bool lead = WAVE ? t % WAVE_GROUP == 0 : true;
+    for (long j = WAVE ? t / WAVE_GROUP : t; j < m; j += nw) {
       const long i = list[j];
       int ix, iy, iz;
       b.at(i, ix, iy, iz);
@@ -696,13 +696,13 @@ struct OpSweepFloor {
   const unsigned *list, *cnt;
   Box b;
   long nw;
-  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
+  KD_HD long size() const { return WAVE ? (long)WAVE_GROUP * nw : nw; }
   KD_INL void operator()(long t) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
     const long m = (long)*cnt;
-    const bool lead = WAVE ? (t & 63) == 0 : true;
-    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
+    const bool lead = WAVE ? t % WAVE_GROUP == 0 : true;
+    for (long j = WAVE ? t / WAVE_GROUP : t; j < m; j += nw) {
       const long i = list[j];
       int ix, iy, iz;
       b.at(i, ix, iy, iz);
@@ -1100,23 +1100,23 @@ struct OpU2P {
   }
 };
 
-// the cells OpU2P left, with the full calc_primitives.  WAVE: one wave per
-// cell (the W inversions evaluate their trials one per lane), otherwise one
-// lane per cell; nw waves (or lanes) stride over the list, whose length is
-// known only on the device
+// the cells OpU2P left, with the full calc_primitives.  WAVE: a group of
+// WAVE_GROUP lanes per cell (the W inversions evaluate their trials one per
+// lane), otherwise one lane per cell; nw groups (or lanes) stride over the
+// list, whose length is known only on the device
 template <int MAG, int RAD, bool WAVE = true>
 struct OpU2PFloor {
   static constexpr int kind = K_U2P_FLOOR;
   static constexpr int waves_per_simd = 2;
   OpU2P<MAG, RAD> op;
-  long nw;  // waves (WAVE: one cell per wave) or lanes (one cell per lane) in flight
-  KD_HD long size() const { return WAVE ? 64 * nw : nw; }
+  long nw;  // lane groups (WAVE: one cell per group) or lanes (one cell per lane) in flight
+  KD_HD long size() const { return WAVE ? (long)WAVE_GROUP * nw : nw; }
   KD_INL Cnt3 operator()(long t) const
   {
     Cnt3 sum{{0, 0, 0}};
     const long m = (long)*op.cnt;
-    const bool lead = WAVE ? (t & 63) == 0 : true;
-    for (long j = WAVE ? t >> 6 : t; j < m; j += nw) {
+    const bool lead = WAVE ? t % WAVE_GROUP == 0 : true;
+    for (long j = WAVE ? t / WAVE_GROUP : t; j < m; j += nw) {
       const long i = (long)op.list[j];
       const Cnt3 c = op.why[j] ? op.template cell<false, true, WAVE>(i, nullptr, nullptr, lead)
                                 : op.template cell<false, false, WAVE>(i, nullptr, nullptr, lead);
@@ -1281,7 +1281,7 @@ struct GridTune {
   int block[TUNE_KINDS];  // threads per block of each operator kind: 64, 128 or 256 (0: 256)
   int fused[3];         // sweep of direction d: 1 reconstruction inside the flux kernel
                         // (OpSweep), 0 face states through HBM (OpRecon + OpFlux)
-  int floor_wave;       // deferred passes: 1 one wave per face or cell, 0 one lane
+  int floor_wave;       // deferred passes: 1 a group of WAVE_GROUP lanes per face or cell, 0 one lane
   int update_pre;       // update: 1 the cell's u and p loaded with the fluxes
   int comb_nv;          // 1: stage algebra with the variable count a template parameter
   int u2p_budget;       // OpU2P::budget
