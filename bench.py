@@ -471,22 +471,27 @@ def bench_grid(args, kind):
     # on a few cells (u2p): not candidates for the dominant kernel
     dom = max((k for k in split if not k.endswith("_floor")), key=lambda k: split[k][0])
     # timed steps, with the time kept on the device (no host round trip
-    # between steps; dt of each step from the previous one, as ko.c does)
-    # and HIP events around the dominant operator's launches only;
-    # --graph replays one captured HIP graph per step instead (about 2 %
-    # faster on MI355X, but with no events around the dominant kernel)
-    if args.graph:
+    # between steps; dt of each step from the previous one, as ko.c does):
+    # all but the last replay one captured HIP graph per step (the graph is
+    # captured and replayed once before the timed region); the last runs as
+    # stream launches with HIP events around the dominant operator's
+    # launches, which give its duration.  --eager times every step that way.
+    graph = not args.eager and args.steps > 1
+    if graph:
         dg.use_graph(True)
         dg.run(1)
-    else:
-        dg.timing_only([dom])
-        dg.timing(enable=True, read=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    dg.run(args.steps)
+    counters = np.zeros(6, np.int64)
+    if graph:
+        dg.run(args.steps - 1)
+        counters += np.asarray(dg.counters, np.int64)
+    dg.timing_only([dom])
+    dg.timing(enable=True, read=False)
+    dg.run(1 if graph else args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    counters = np.asarray(dg.counters, np.int64)
+    counters += np.asarray(dg.counters, np.int64)
     tm = dg.timing(enable=False)
     dg.timing_only(None)
     if dom not in tm:
@@ -583,6 +588,9 @@ def bench_grid(args, kind):
                                      "device fault of an earlier build); this line is the GRMHD step")
                    if kind == "grmhd" else None,
                    "counters_per_step": (counters / args.steps).tolist(),
+                   "timing": ("every step as stream launches, HIP events around the dominant operator" if not graph
+                              else "steps 1..K-1 replay one captured HIP graph per step; step K as stream launches "
+                                   "with HIP events around the dominant operator's launches"),
                    "operator_ms_per_step_pilot": {k: v[0] / 3 for k, v in sorted(split.items(), key=lambda kv: -kv[1][0])}},
         "roofline": roof,
     }
@@ -662,7 +670,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="grid workloads: every timed step as stream launches (default: all but the last replay a "
+                         "captured HIP graph)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
     ap.add_argument("--tune-json", default=None,
                     help="launch choices to use instead of the search, as the line's config.autotune reports "
