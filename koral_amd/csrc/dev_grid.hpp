@@ -328,7 +328,7 @@ KD_INL void ws_core(const double *ucon, const Geom &g, double w2, int dim, doubl
 
 // Per-cell record: ws[k] for k = 0..5 gas l/r per dim, 6..11 radiation l/r,
 // 12..14 gas max |a| per dim, 15..17 radiation max (save_wavespeeds
-// finite.c:296-398).
+// finite.c:296-398).  Without radiation the radiation slots are not stored.
 constexpr int NWS = 18;
 
 template <int MAG, int RAD>

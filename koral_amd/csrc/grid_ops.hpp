@@ -340,7 +340,9 @@ struct OpWavespeeds {
     geom_cell(G, ix, iy, g);
     wavespeeds_cell<MAG, RAD>(P, G, pp, ix, iy, iz, g, a);
 #pragma unroll
-    for (int k = 0; k < NWS; k++) ws[k * G.ncell + c] = a[k];
+    for (int k = 0; k < NWS; k++)
+      // without radiation only the gas slots (0..5, 12..14) are read
+      if (RAD || k < 6 || (k >= 12 && k < 15)) ws[k * G.ncell + c] = a[k];
     if (out != 0) return 0.;
     double w[3];
 #pragma unroll

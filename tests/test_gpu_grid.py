@@ -101,9 +101,11 @@ def test_grid_wavespeeds_fluxes(oracle, kind):
     tm_o, ws_o, fl_o = S.explicit_parts(0., dt)
     dg.explicit(0., dt)
     ws_d, fl_d = dg.work()
-    # wave speeds are written on the cells the sweep uses; compare there
+    # wave speeds are written on the cells the sweep uses; compare there,
+    # the radiation slots only with radiation (the device stores them then)
     used = np.any(ws_o != 0., axis=-1) | np.any(ws_d != 0., axis=-1)
-    rw = _check("ws", np.where(used[..., None], ws_d, 0.), ws_o, P)
+    slots = list(range(18)) if P.rad else [0, 1, 2, 3, 4, 5, 12, 13, 14]
+    rw = _check("ws", np.where(used[..., None], ws_d, 0.)[..., slots], ws_o[..., slots], P)
     worst = 0.
     for d in range(3):
         if fl_o[d] is None:
