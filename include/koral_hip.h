@@ -355,6 +355,21 @@ int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t);
    calc_primitives pass (also after a step; a step reuses the face counts).
    Synchronises the grid's stream. */
 int koral_hip_grid_deferred_counts(koral_hip_grid *g, unsigned counts[4]);
+/* Learned Newton starts for the grid's implicit operator (no reference
+   counterpart; for callers that train the networks of the learned batch
+   solve): every later op_implicit, and the implicit stages of the RK2IMEX
+   step, solve each interior cell as koral_hip_implicit_batch_learned_gated
+   does -- starts from the gas (and radiation) network on the old state,
+   cells the gate (may be NULL) scores below thr start the plain chain at
+   once, the learned tries limited to maxiter iterations, the unchanged
+   chain for cells the tries leave.  gas NULL restores the plain solve.  The
+   grid needs radiation; the networks must outlive their use. */
+int koral_hip_grid_set_learned(koral_hip_grid *g, koral_hip_mlp *gas, koral_hip_mlp *rad, koral_hip_mlp *gate,
+                               float thr, int maxiter);
+/* The starts of the last learned implicit operator, [4][n] over the
+   interior cells in x-fastest order (gxr may be NULL; NaN gas starts mark
+   cells the gate left to the plain chain).  Synchronises. */
+int koral_hip_grid_download_guess(koral_hip_grid *g, double *gxg, double *gxr);
 int koral_hip_grid_set_tune(koral_hip_grid *g, const koral_grid_tune *t);
 /* Coordinate search over the choices above on the grid's current state at
    time t with step dt: each candidate runs 1 + steps RK2IMEX steps from a

@@ -164,9 +164,40 @@ struct GridCells {
   }
 };
 
+// Learned Newton starts of the grid's implicit solve (no reference
+// counterpart): the networks, the gate threshold, the learned budget, and
+// work arrays over the n interior cells: primitives and dt gathered in
+// batch layout [NV][n] and [n], features [11][n], outputs [4][n], the
+// starts [4][n] and the second try's lists.
+struct GridLearned {
+  koral_hip_mlp *gas = nullptr, *rad = nullptr, *gate = nullptr;
+  float thr = 0.f;
+  int maxiter = 0;
+  long n = 0;
+  double *pb = nullptr, *dtb = nullptr, *gxg = nullptr, *gxr = nullptr;
+  float *feat = nullptr, *y = nullptr;
+  unsigned *second = nullptr;
+};
+
+// interior primitives and the solve's time step in batch layout
+__global__ void k_gather_interior(DGrid G, int nv, const double *p, const double *clk, double fac, double *pb,
+                                  double *dtb)
+{
+  const long n = (long)G.nx * G.ny * G.nz;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Box b{0, 0, 0, G.nx, G.ny, G.nz};
+  int ix, iy, iz;
+  b.at(i, ix, iy, iz);
+  const long c = cidx(G, ix, iy, iz);
+  for (int iv = 0; iv < nv; iv++) pb[iv * n + i] = p[iv * G.ncell + c];
+  dtb[i] = clk[1] * fac;
+}
+
 struct HipExec {
   hipStream_t s;
   Timing *tm;
+  const GridLearned *lrn = nullptr;  // learned starts of the implicit solve, when set
   const GridTune *tune = nullptr;  // threads per block of each operator kind
   int rc = 0;  // first launch-configuration error
   template <class Op>
@@ -249,7 +280,29 @@ struct HipExec {
   {
     timed(op, [&](long n) {
       GridCells<MAG> src{op.G, op.u, op.p, op.fl, op.clk, op.fac};
-      int r = implicit_passes<MAG>(op.P, src, n, imp, out, s);
+      int r = 0;
+      if (lrn && lrn->gas) {
+        // the starts from the networks on the old state, then the passes
+        // with the learned tries first (as the batch learned solve)
+        const GridLearned &L = *lrn;
+        const int nv = op.P.nv;
+        k_gather_interior<<<blocks_for(n), BS, 0, s>>>(op.G, nv, op.p, op.clk, op.fac, L.pb, L.dtb);
+        r = koral_hip_mlp_features(&op.P, L.gas, n, L.pb, L.dtb, L.feat, s);
+        if (!r) r = koral_hip_mlp_infer(L.gas, L.feat, n, L.y, s);
+        if (!r) r = koral_hip_mlp_guess(L.gas, n, nv, L.pb, L.y, L.gxg, s);
+        if (!r && L.rad) r = koral_hip_mlp_features(&op.P, L.rad, n, L.pb, L.dtb, L.feat, s);
+        if (!r && L.rad) r = koral_hip_mlp_infer(L.rad, L.feat, n, L.y, s);
+        if (!r && L.rad) r = koral_hip_mlp_guess(L.rad, n, nv, L.pb, L.y, L.gxr, s);
+        if (!r && L.gate) r = koral_hip_mlp_features(&op.P, L.gate, n, L.pb, L.dtb, L.feat, s);
+        if (!r && L.gate) r = koral_hip_mlp_infer(L.gate, L.feat, n, L.y, s);
+        if (!r && L.gate) r = koral_hip_mlp_gate(L.gate, n, L.y, L.thr, L.gxg, s);
+        ImpBufs b2 = imp;
+        b2.second[0] = L.second;
+        b2.second[1] = L.second + n;
+        if (!r) r = implicit_passes<MAG>(op.P, src, n, b2, out, s, L.gxg, L.rad ? L.gxr : nullptr, L.maxiter);
+      } else {
+        r = implicit_passes<MAG>(op.P, src, n, imp, out, s);
+      }
       if (r && !rc) rc = r;
     });
   }
@@ -300,6 +353,7 @@ struct koral_hip_grid {
   hipStream_t side[3] = {nullptr, nullptr, nullptr};  // default priority, non-blocking
   hipEvent_t ev_fork[3] = {nullptr, nullptr, nullptr}, ev_join[3] = {nullptr, nullptr, nullptr};
   Timing timing;
+  GridLearned lrn;
   std::vector<void *> allocs;
   double *stage = nullptr;  // AoS staging for upload/download
   // one RK2IMEX step plus the clock advance, captured once and replayed by
@@ -351,6 +405,7 @@ struct koral_hip_grid {
   {
     HipExec e{s, &timing};
     e.tune = &b.tune;
+    e.lrn = &lrn;
     for (int k = 0; k < 3; k++) {
       e.side[k] = side[k];
       e.ev_fork[k] = ev_fork[k];
@@ -952,6 +1007,53 @@ int tune_trial(koral_hip_grid *g, TuneSnap &snap, int steps, hipEvent_t e0, hipE
 }  // namespace
 
 extern "C" {
+
+int koral_hip_grid_set_learned(koral_hip_grid *g, koral_hip_mlp *gas, koral_hip_mlp *rad, koral_hip_mlp *gate,
+                               float thr, int maxiter)
+{
+  if (!g) return set_error(KORAL_E_ARG, "grid_set_learned: null grid");
+  GridLearned &L = g->lrn;
+  g->drop_graph();  // a captured step holds the other implicit path
+  if (!gas) {
+    L.gas = L.rad = L.gate = nullptr;
+    return 0;
+  }
+  if (!g->b.P.rad) return set_error(KORAL_E_ARG, "grid_set_learned: the grid has no radiation");
+  if (koral_hip_mlp_kind(gas) != 0 || (rad && koral_hip_mlp_kind(rad) != 1) || (gate && koral_hip_mlp_kind(gate) != 3))
+    return set_error(KORAL_E_ARG, "grid_set_learned: networks of the wrong kind");
+  if (maxiter < 0) return set_error(KORAL_E_ARG, "grid_set_learned: maxiter must be >= 0");
+  const DGrid &G = g->b.G;
+  const long n = (long)G.nx * G.ny * G.nz;
+  if (L.n != n) {
+    int rc = g->alloc(&L.pb, (size_t)g->b.P.nv * n);
+    if (!rc) rc = g->alloc(&L.dtb, (size_t)n);
+    if (!rc) rc = g->alloc(&L.gxg, (size_t)4 * n);
+    if (!rc) rc = g->alloc(&L.gxr, (size_t)4 * n);
+    if (!rc) rc = g->alloc(&L.feat, (size_t)11 * n);
+    if (!rc) rc = g->alloc(&L.y, (size_t)4 * n);
+    if (!rc) rc = g->alloc(&L.second, (size_t)2 * n);
+    if (rc) return rc;
+    L.n = n;
+  }
+  L.gas = gas;
+  L.rad = rad;
+  L.gate = gate;
+  L.thr = thr;
+  L.maxiter = maxiter;
+  return 0;
+}
+
+int koral_hip_grid_download_guess(koral_hip_grid *g, double *gxg, double *gxr)
+{
+  if (!g || !gxg) return set_error(KORAL_E_ARG, "grid_download_guess: bad arguments");
+  const GridLearned &L = g->lrn;
+  if (!L.gas || !L.n) return set_error(KORAL_E_STATE, "grid_download_guess: no learned starts set");
+  const size_t bytes = (size_t)4 * L.n * sizeof(double);
+  int rc = check_hip(hipMemcpyAsync(gxg, L.gxg, bytes, hipMemcpyDeviceToHost, g->s), "download gas starts");
+  if (!rc && gxr && L.rad) rc = check_hip(hipMemcpyAsync(gxr, L.gxr, bytes, hipMemcpyDeviceToHost, g->s), "download rad starts");
+  if (!rc) rc = check_hip(hipStreamSynchronize(g->s), "download starts sync");
+  return rc;
+}
 
 int koral_hip_grid_deferred_counts(koral_hip_grid *g, unsigned counts[4])
 {

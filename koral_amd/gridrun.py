@@ -59,6 +59,8 @@ def _sig(L):
         "koral_hip_grid_read_restart": [_vp, ctypes.c_char_p, i, _vp, _vp, _vp],
         "koral_hip_grid_get_tune": [_vp, _vp],
         "koral_hip_grid_deferred_counts": [_vp, _vp],
+        "koral_hip_grid_set_learned": [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_int],
+        "koral_hip_grid_download_guess": [_vp, _vp, _vp],
         "koral_hip_grid_set_tune": [_vp, _vp],
         "koral_hip_grid_autotune": [_vp, d, d, i, ctypes.c_char_p, i],
     }.items():
@@ -226,6 +228,25 @@ class DeviceGrid:
         check(r, "read_restart")
         self.t = t.value
         return True, n1.value
+
+    def set_learned(self, gas, rad=None, gate=None, thr=0., maxiter=16):
+        """Learned Newton starts for the implicit operator (koral_amd.learn.Net
+        networks of kinds gas, radiation, gate; gas None: the plain solve).
+        The networks must stay alive while the grid uses them."""
+        h = lambda net: net.h if net is not None else None
+        check(self.L.koral_hip_grid_set_learned(self.h, h(gas), h(rad), h(gate), float(thr), int(maxiter)),
+              "grid_set_learned")
+        self._nets = (gas, rad, gate)
+
+    def download_guess(self):
+        """The starts of the last learned implicit operator: gas and radiation
+        [4][n] over the interior cells in x-fastest order."""
+        n = self.spec.nx * self.spec.ny * self.spec.nz
+        gxg = np.zeros((4, n))
+        gxr = np.zeros((4, n))
+        check(self.L.koral_hip_grid_download_guess(self.h, gxg.ctypes.data_as(_vp), gxr.ctypes.data_as(_vp)),
+              "grid_download_guess")
+        return gxg, gxr
 
     def deferred_counts(self):
         """[x1, x2, x3 floor faces of the last explicit operator, cells left

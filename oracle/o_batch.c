@@ -217,6 +217,12 @@ int oracle_grid_implicit(const oracle_params *P, const o_grid *G, double t, doub
   return o_op_implicit(P, G, t, dt, u, p, flags, counts);
 }
 
+int oracle_grid_implicit_guess(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                               int *flags, int *counts, const double *gxg, const double *gxr, int maxiter)
+{
+  return o_op_implicit_guess(P, G, t, dt, u, p, flags, counts, gxg, gxr, maxiter);
+}
+
 int oracle_grid_do_correct(const oracle_params *P, const o_grid *G, double *u, double *p)
 {
   return o_do_correct(P, G, u, p);

@@ -969,6 +969,16 @@ int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, in
 int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                   int *counts)
 {
+  return o_op_implicit_guess(P, G, t, dt, u, p, flags, counts, NULL, NULL, 0);
+}
+
+/* op_implicit with learned Newton starts (no reference counterpart): gxg,
+   gxr [cell][4] over the interior cells in x-fastest order, each solved by
+   o_solve_implicit_lab_guess with the iteration budget maxiter; NULL gxg is
+   op_implicit itself. */
+int o_op_implicit_guess(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                        int *flags, int *counts, const double *gxg, const double *gxr, int maxiter)
+{
   if (!P->rad) return 0;
   int nv = P->nv;
   int c0 = 0, c1 = 0, c2 = 0;
@@ -980,7 +990,9 @@ int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, 
         o_geom g;
         o_geom_cell(G, ix, iy, &g);
         o_imp_info inf;
-        int r = o_solve_implicit_lab(P, u + (long)nv * c, p + (long)nv * c, &g, dt, t, flags + O_NFLAGS * c, &inf);
+        const long i = ix + (long)G->nx * (iy + (long)G->ny * iz);
+        int r = o_solve_implicit_lab_guess(P, u + (long)nv * c, p + (long)nv * c, &g, dt, t, flags + O_NFLAGS * c,
+                                           &inf, gxg ? gxg + 4 * i : NULL, gxr ? gxr + 4 * i : NULL, maxiter);
         c0 += (r == 0);
         c1 += inf.iters;
         c2 += (r != 0);

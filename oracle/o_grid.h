@@ -27,6 +27,8 @@ int o_op_explicit_parts(const oracle_params *P, const o_grid *G, double t, doubl
                         double *tstepdenmax, double *ws_out, double **flb_out);
 int o_cell_fixup(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int type);
 int o_calc_u2p(const oracle_params *P, const o_grid *G, double *u, double *p, int *flags, int *counts);
+int o_op_implicit_guess(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p,
+                        int *flags, int *counts, const double *gxg, const double *gxr, int maxiter);
 int o_op_implicit(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,
                   int *counts);
 int o_step_rk2imex(const oracle_params *P, const o_grid *G, double t, double dt, double *u, double *p, int *flags,

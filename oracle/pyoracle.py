@@ -228,6 +228,8 @@ def _grid_lib():
         L.oracle_grid_u2p.argtypes = [vp, vp, _dp, _dp, _ip, _ip]
         L.oracle_grid_cell_fixup.argtypes = [vp, vp, _dp, _dp, _ip, ctypes.c_int]
         L.oracle_grid_implicit.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
+        L.oracle_grid_implicit_guess.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip,
+                                                 _dp, _dp, ctypes.c_int]
         L.oracle_grid_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _dp, _ip]
         L.oracle_grid_do_correct.argtypes = [vp, vp, _dp, _dp]
         L.oracle_grid_set_threads.argtypes = [ctypes.c_int]
@@ -301,10 +303,20 @@ class GridState:
         """do_correct: the polar-axis correction when P.polar_nc > 0."""
         _grid_lib().oracle_grid_do_correct(_P(self.P), ctypes.byref(self.grid.c), _d(self.u), _d(self.p))
 
-    def implicit(self, t, dt):
+    def implicit(self, t, dt, gxg=None, gxr=None, maxiter=0):
+        """op_implicit; with gxg (and gxr) [4][n] learned starts over the
+        interior cells in x-fastest order, each cell solved from them as
+        o_solve_implicit_lab_guess does (maxiter the learned budget)."""
         c = np.zeros(3, np.int32)
-        _grid_lib().oracle_grid_implicit(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
-                                         _i(self.flags), _i(c))
+        if gxg is None:
+            _grid_lib().oracle_grid_implicit(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u), _d(self.p),
+                                             _i(self.flags), _i(c))
+            return c
+        gg = np.ascontiguousarray(np.asarray(gxg, np.float64).T)
+        gr = None if gxr is None else np.ascontiguousarray(np.asarray(gxr, np.float64).T)
+        _grid_lib().oracle_grid_implicit_guess(_P(self.P), ctypes.byref(self.grid.c), t, dt, _d(self.u),
+                                               _d(self.p), _i(self.flags), _i(c), _d(gg),
+                                               _d(gr) if gr is not None else None, int(maxiter))
         return c
 
     def step(self, t, dt):

@@ -239,3 +239,32 @@ def test_threaded_step_is_bit_identical(oracle, kind):
     for x, y in zip(a[:3], b[:3]):
         assert np.array_equal(x, y)
     assert a[3] == b[3] and a[4] == b[4]
+
+
+def test_implicit_with_gated_starts_is_plain(oracle):
+    """The grid implicit operator with learned starts: cells whose gas start
+    is NaN (left out by the gate) take the unchanged chain, so all-NaN starts
+    give the plain operator bit for bit; finite starts are used (the state
+    differs from the plain one somewhere)."""
+    P = KP.params_torus_rad()
+    spec, p = PR.torus_setup(P, n=(8, 8, 4))
+    p = GR.to_aos(p)
+    n = spec.nx * spec.ny * spec.nz
+    out = []
+    for gx in (None, np.full((4, n), np.nan), "old"):
+        S = oracle.GridState(P, oracle.Grid(spec), p)
+        S.init_u()
+        S.set_bc()
+        dt = 1. / S.tstepden()
+        if isinstance(gx, str):
+            # starts at 1.01 times the old gas energy, the old velocities: a
+            # different Newton path from the plain one
+            q = S.p.reshape(spec.sz, spec.sy, spec.sx, P.nv)[spec.ngz:spec.ngz + spec.nz, spec.ngy:spec.ngy + spec.ny,
+                                                           spec.ngx:spec.ngx + spec.nx]
+            gx = np.ascontiguousarray(np.moveaxis(q[..., 1:5], -1, 0).reshape(4, n))
+            gx[0] *= 1.01
+        S.implicit(0., dt, gx, None, maxiter=16)
+        out.append((S.p.copy(), S.u.copy(), S.flags.copy()))
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert all(same(x, y) for x, y in zip(out[0], out[1]))
+    assert not same(out[0][0], out[2][0])
