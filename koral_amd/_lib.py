@@ -22,8 +22,8 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIBPATH):
-        raise ImportError("libkoral_hip.so is not built (run __graft_entry__.build() or "
-                          "python koral_amd/build.py); the HIP path has no CPU fallback")
+        raise ImportError("libkoral_hip.so is not built (run python koral_amd/build.py); "
+                          "the HIP path has no CPU fallback")
     # torch ships its own HIP runtime under the same soname; loading it first
     # makes this library bind to that one, so a process never holds two HIP
     # runtimes (the second would leave torch without a device).
