@@ -337,10 +337,11 @@ KD_INL bool w_bad(double W, double Bsq, double QdotBsq, double Qtsq)
 constexpr int WAVE_GROUP = 64;  // lanes per point of the WAVE solver (divides 64)
 static_assert(64 % WAVE_GROUP == 0, "WAVE_GROUP divides the wave");
 constexpr int U2P_OVER_BUDGET = -200;
-template <int MAG, int RAD, bool WAVE = false>
+template <int MAG, int RAD, bool WAVE = false, int WG = WAVE_GROUP>
 KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, const Geom &g, int etype,
                         double Wstart = 0., int *iters = nullptr, int budget = 0)
 {
+  static_assert(64 % WG == 0, "a lane group divides the wave");
   double gdetu = g.gdet;
   double alpha = sqrt(-1. / g.G[0][0]);
   double D = uu[0] / gdetu * alpha;
@@ -391,11 +392,11 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
   if (iters) *iters = 0;
 #if KD_WAVE_SOLVER
   if constexpr (WAVE) {
-    const int lane = threadIdx.x & 63, sub = lane % WAVE_GROUP, g0 = lane - sub;
-    const unsigned long long gmask = (WAVE_GROUP == 64) ? ~0ull : (((1ull << WAVE_GROUP) - 1ull) << g0);
+    const int lane = threadIdx.x & 63, sub = lane % WG, g0 = lane - sub;
+    const unsigned long long gmask = (WG == 64) ? ~0ull : (((1ull << WG) - 1ull) << g0);
     bool found = false;
-    for (int r = 0; r * WAVE_GROUP < 50 && !found; r++) {
-      const int k = r * WAVE_GROUP + sub;
+    for (int r = 0; r * WG < 50 && !found; r++) {
+      const int k = r * WG + sub;
       double Wk = W;
       for (int q = 0; q < k; q++) Wk *= 10.;
       bool ok = false;
@@ -456,12 +457,12 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
     double Wnew = W - f0 / dfdW;
 #if KD_WAVE_SOLVER
     if constexpr (WAVE) {
-      // trials k = 0..99 in rounds of WAVE_GROUP; trial k halves the step k times
-      const int lane = threadIdx.x & 63, sub = lane % WAVE_GROUP, g0 = lane - sub;
-      const unsigned long long gmask = (WAVE_GROUP == 64) ? ~0ull : (((1ull << WAVE_GROUP) - 1ull) << g0);
+      // trials k = 0..99 in rounds of WG; trial k halves the step k times
+      const int lane = threadIdx.x & 63, sub = lane % WG, g0 = lane - sub;
+      const unsigned long long gmask = (WG == 64) ? ~0ull : (((1ull << WG) - 1ull) << g0);
       int found = -1;
-      for (int round = 0; round * WAVE_GROUP < 100 && found < 0; round++) {
-        const int k = round * WAVE_GROUP + sub;
+      for (int round = 0; round * WG < 100 && found < 0; round++) {
+        const int k = round * WG + sub;
         // k halvings of 1 are exact: 2^-k
         const double dumpfac = ldexp(1., -k);
         const double Wt = k == 0 ? Wnew : W - dumpfac * f0 / dfdW;
@@ -474,7 +475,7 @@ KD_FUN int u2p_solver_W(const koral_params &P, const double *uu, double *pp, con
         const unsigned long long m = __ballot(ok) & gmask;
         if (m) {
           const int win = __ffsll(m) - 1;
-          found = round * WAVE_GROUP + win - g0;
+          found = round * WG + win - g0;
           Wnew = __shfl(Wt, win);
           fc = __shfl(f0t, win);
           dfc = __shfl(dft, win);
@@ -787,7 +788,7 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
 // two inlined Newton inversions of that rare branch out of a kernel's
 // common path is what DEFER is for.
 constexpr int FLOORS_DEFERRED = 1;
-template <int MAG, int RAD, bool DEFER = false, bool WAVE = false>
+template <int MAG, int RAD, bool DEFER = false, bool WAVE = false, int WG = WAVE_GROUP>
 KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
@@ -833,8 +834,8 @@ KD_FUN int floors_mhd(const koral_params &P, double *pp, const Geom &g)
       p2u_mhd<MAG, RAD>(P, dpp, duu, g);
 #pragma unroll
       for (int i = 0; i < 9; i++) uu[i] += duu[i];
-      int rt = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0);
-      if (rt < 0) rt = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 1);
+      int rt = u2p_solver_W<MAG, RAD, WAVE, WG>(P, uu, pp, g, 0);
+      if (rt < 0) rt = u2p_solver_W<MAG, RAD, WAVE, WG>(P, uu, pp, g, 1);
       ret = -1;
     }
   }

@@ -591,6 +591,35 @@ struct OpFlux {
   }
 };
 
+// The face floors of both states of a face.  WAVE: the two halves of the
+// wave floor the left and the right state at once (each half a 32-lane
+// group of the wave-parallel W inversion), then every lane takes both
+// results by shuffle; the two states are floored independently, so the
+// results are the sequential ones.
+template <int MAG, int RAD, bool WAVE>
+KD_INL void floors_pair(const koral_params &P, double *pL, double *pR, const Geom &g)
+{
+  constexpr int NV = Lay<MAG, RAD>::NV;
+#if KD_WAVE_SOLVER
+  if constexpr (WAVE) {
+    const int lane = threadIdx.x & 63;
+    const bool right = lane >= 32;
+    double ps[NV];
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) ps[iv] = right ? pR[iv] : pL[iv];
+    floors_mhd<MAG, RAD, false, true, 32>(P, ps, g);
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) {
+      pL[iv] = __shfl(ps[iv], lane & 31);
+      pR[iv] = __shfl(ps[iv], 32 + (lane & 31));
+    }
+    return;
+  }
+#endif
+  floors_mhd<MAG, RAD, false, WAVE>(P, pL, g);
+  floors_mhd<MAG, RAD, false, WAVE>(P, pR, g);
+}
+
 // The faces OpFlux left: the same face states, the full face floors, the
 // same fluxes.  WAVE: a group of WAVE_GROUP lanes per face (the floors' W
 // inversions evaluate their trials one per lane; the group's first lane
@@ -621,8 +650,7 @@ struct OpFluxFloor {
       load_face_states<NV>(fs, b.size(), i, pL, pR);
       Geom g;
       geom_face(G, D, ix, iy, g);
-      floors_mhd<MAG, RAD, false, WAVE>(P, pL, g);
-      floors_mhd<MAG, RAD, false, WAVE>(P, pR, g);
+      floors_pair<MAG, RAD, WAVE>(P, pL, pR, g);
       face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g, lead);
     }
   }
@@ -712,8 +740,7 @@ struct OpSweepFloor {
       face_states<MAG, RAD, ORDER, D>(P, G, p, ix, iy, iz, pL, pR);
       Geom g;
       geom_face(G, D, ix, iy, g);
-      floors_mhd<MAG, RAD, false, WAVE>(P, pL, g);
-      floors_mhd<MAG, RAD, false, WAVE>(P, pR, g);
+      floors_pair<MAG, RAD, WAVE>(P, pL, pR, g);
       face_flux<MAG, RAD, D>(P, G, ws, flb, ix, iy, iz, pL, pR, g, lead);
     }
   }
