@@ -440,7 +440,7 @@ def bench_grid(args, kind):
         names = {dg.L.koral_hip_grid_op_name(k).decode(): k for k in range(32)}
         for d, v in enumerate(want.get("fused", list(tn.fused))):
             tn.fused[d] = int(v)
-        for key in ("floor_wave", "update_pre", "comb_nv", "u2p_budget", "floor_nw", "u2p_nw"):
+        for key in ("floor_wave", "update_pre", "comb_nv", "u2p_budget", "u2p_group", "floor_nw", "u2p_nw"):
             if key in want:
                 setattr(tn, key, int(want[key]))
         for nm, b in want.get("block", {}).items():
@@ -460,7 +460,7 @@ def bench_grid(args, kind):
         tune_info = {"seconds": time.perf_counter() - t0, "trials": len(rows) - len(chosen),
                      "start_ms_per_step": rows[0][2], "end_ms_per_step": chosen[-1][2] if chosen else rows[0][2],
                      "fused": list(tn.fused), "floor_wave": tn.floor_wave, "update_pre": tn.update_pre,
-                     "comb_nv": tn.comb_nv, "u2p_budget": tn.u2p_budget,
+                     "comb_nv": tn.comb_nv, "u2p_budget": tn.u2p_budget, "u2p_group": tn.u2p_group,
                      "floor_nw": tn.floor_nw,
                      "u2p_nw": tn.u2p_nw,
                      "block": {names[k]: tn.block[k] for k in range(32) if names[k] and tn.block[k] != 256}}
@@ -681,7 +681,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
     ap.add_argument("--tune-json", default=None,
                     help="launch choices to use instead of the search, as the line's config.autotune reports "
-                         "them (fused, floor_wave, update_pre, comb_nv, u2p_budget, floor_nw, u2p_nw, block)")
+                         "them (fused, floor_wave, update_pre, comb_nv, u2p_budget, u2p_group, floor_nw, u2p_nw, "
+                         "block)")
     ap.add_argument("--tune-steps", type=int, default=3)
     ap.add_argument("--advance", type=int, default=0,
                     help="grid workloads: untimed steps before the tuning and the timed steps")

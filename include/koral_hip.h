@@ -345,6 +345,8 @@ typedef struct koral_grid_tune {
   int u2p_budget;     /* residual evaluations the common u2p pass spends on
                          a cell's hot inversion before leaving the cell to
                          the deferred (wave-parallel) pass; 0: no bound */
+  int u2p_group;      /* lanes per cell of the deferred u2p pass with
+                         floor_wave = 1: 64 (one cell per wave) or 32 */
   long floor_nw, u2p_nw;
 } koral_grid_tune;
 int koral_hip_grid_get_tune(koral_hip_grid *g, koral_grid_tune *t);

@@ -738,7 +738,7 @@ KD_INL double ff_Rtt(const double *pp, const Geom &g, double *ucon_out)
 // marks the other cases, where the redo runs the hot inversion again: -105,
 // and a hot inversion stopped by the budget (DEFER only, see u2p_solver_W).
 constexpr int U2P_DEFERRED = 1, U2P_DEFERRED_105 = 2;
-template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false, int WG = WAVE_GROUP>
 KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom &g, int *corrected, int *fixups,
                int budget = 0)
 {
@@ -759,11 +759,11 @@ KD_FUN int u2p(const koral_params &P, const double *uu0, double *pp, const Geom 
     uu[0] = P.rhofloor * gdetu;
     ret = -2;
   }
-  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 0, 0., nullptr, DEFER ? budget : 0);
+  u2pret = SKIP_HOT ? -1 : u2p_solver_W<MAG, RAD, WAVE, WG>(P, uu, pp, g, 0, 0., nullptr, DEFER ? budget : 0);
   if (u2pret < 0) {
     if (DEFER) return (u2pret == -105 || u2pret == U2P_OVER_BUDGET) ? U2P_DEFERRED_105 : U2P_DEFERRED;
     ret = -1;
-    u2pret = u2p_solver_W<MAG, RAD, WAVE>(P, uu, pp, g, 1);
+    u2pret = u2p_solver_W<MAG, RAD, WAVE, WG>(P, uu, pp, g, 1);
     if (u2pret < 0) ret = -2;
   }
   if (u2pret < 0) {
@@ -898,19 +898,19 @@ KD_FUN int floors_rad(const koral_params &P, double *pp, const Geom &g)
 // or CP_DEFERRED before writing uu (pp may be changed) and is redone by the
 // caller with DEFER false, with SKIP_HOT for CP_DEFERRED_HOT.
 constexpr int CP_DEFERRED = 100, CP_DEFERRED_HOT = 101;
-template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false>
+template <int MAG, int RAD, bool DEFER = false, bool SKIP_HOT = false, bool WAVE = false, int WG = WAVE_GROUP>
 KD_FUN int calc_primitives(const koral_params &P, double *uu, double *pp, const Geom &g, int &fl_entr,
                                int &fl_entr2, int &fl_hd, int &fl_rad, int budget = 0)
 {
   int corrected[3] = {0, 0, 0}, fixups[2] = {0, 0};
   fl_entr = 0;
   fl_entr2 = 0;
-  int r = u2p<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, corrected, fixups, budget);
+  int r = u2p<MAG, RAD, DEFER, SKIP_HOT, WAVE, WG>(P, uu, pp, g, corrected, fixups, budget);
   if (DEFER && r == U2P_DEFERRED) return CP_DEFERRED_HOT;
   if (DEFER && r == U2P_DEFERRED_105) return CP_DEFERRED;
   if (corrected[0] == 1) fl_entr = 1;
   if (corrected[2] == 1) fl_entr2 = 1;
-  if (floors_mhd<MAG, RAD, DEFER, WAVE>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;
+  if (floors_mhd<MAG, RAD, DEFER, WAVE, WG>(P, pp, g) == FLOORS_DEFERRED && DEFER) return CP_DEFERRED;
   if (RAD) floors_rad<MAG, RAD>(P, pp, g);
   p2u<MAG, RAD>(P, pp, uu, g);
   fl_hd = (fixups[0] > 0) ? 1 : 0;

@@ -1094,7 +1094,7 @@ struct OpU2P {
   // the deferring calc_primitives, or the full one (optionally from the
   // entropy inversion on) on cell i; WAVE: the calling wave's lanes all run
   // cell i together and only the lane with store writes it
-  template <bool DEFER, bool SKIP_HOT, bool WAVE>
+  template <bool DEFER, bool SKIP_HOT, bool WAVE, int WG = WAVE_GROUP>
   KD_INL Cnt3 cell(long i, unsigned *lst, unsigned *len, bool store) const
   {
     constexpr int NV = Lay<MAG, RAD>::NV;
@@ -1108,7 +1108,7 @@ struct OpU2P {
     Geom g;
     geom_cell(G, ix, iy, g);
     int fe, fe2, fh, fr;
-    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT, WAVE>(P, uu, pp, g, fe, fe2, fh, fr, DEFER ? budget : 0);
+    const int r = calc_primitives<MAG, RAD, DEFER, SKIP_HOT, WAVE, WG>(P, uu, pp, g, fe, fe2, fh, fr, DEFER ? budget : 0);
     if (DEFER) {
       const bool defer = r == CP_DEFERRED || r == CP_DEFERRED_HOT;
       const long slot = append_index(defer, (unsigned)i, lst, len);
@@ -1130,25 +1130,26 @@ struct OpU2P {
 };
 
 // the cells OpU2P left, with the full calc_primitives.  WAVE: a group of
-// WAVE_GROUP lanes per cell (the W inversions evaluate their trials one per
-// lane), otherwise one lane per cell; nw groups (or lanes) stride over the
-// list, whose length is known only on the device
-template <int MAG, int RAD, bool WAVE = true>
+// WG lanes per cell (64: one cell per wave, 32: two), the W inversions
+// evaluating their trials one per lane; otherwise one lane per cell.  nw
+// groups (or lanes) stride over the list, whose length is known only on
+// the device
+template <int MAG, int RAD, bool WAVE = true, int WG = WAVE_GROUP>
 struct OpU2PFloor {
   static constexpr int kind = K_U2P_FLOOR;
   static constexpr int waves_per_simd = 2;
   OpU2P<MAG, RAD> op;
   long nw;  // lane groups (WAVE: one cell per group) or lanes (one cell per lane) in flight
-  KD_HD long size() const { return WAVE ? (long)WAVE_GROUP * nw : nw; }
+  KD_HD long size() const { return WAVE ? (long)WG * nw : nw; }
   KD_INL Cnt3 operator()(long t) const
   {
     Cnt3 sum{{0, 0, 0}};
     const long m = (long)*op.cnt;
-    const bool lead = WAVE ? t % WAVE_GROUP == 0 : true;
-    for (long j = WAVE ? t / WAVE_GROUP : t; j < m; j += nw) {
+    const bool lead = WAVE ? t % WG == 0 : true;
+    for (long j = WAVE ? t / WG : t; j < m; j += nw) {
       const long i = (long)op.list[j];
-      const Cnt3 c = op.why[j] ? op.template cell<false, true, WAVE>(i, nullptr, nullptr, lead)
-                                : op.template cell<false, false, WAVE>(i, nullptr, nullptr, lead);
+      const Cnt3 c = op.why[j] ? op.template cell<false, true, WAVE, WG>(i, nullptr, nullptr, lead)
+                                : op.template cell<false, false, WAVE, WG>(i, nullptr, nullptr, lead);
 #pragma unroll
       for (int k = 0; k < 3; k++) sum.v[k] += c.v[k];
     }
@@ -1314,6 +1315,7 @@ struct GridTune {
   int update_pre;       // update: 1 the cell's u and p loaded with the fluxes
   int comb_nv;          // 1: stage algebra with the variable count a template parameter
   int u2p_budget;       // OpU2P::budget
+  int u2p_group;        // lanes per cell of the wave-mapped deferred u2p pass (64 or 32)
   long floor_nw;        // waves (or lanes) in flight in the deferred face passes
   long u2p_nw;          // waves (or lanes) in flight in the deferred u2p pass
 };
@@ -1326,6 +1328,7 @@ inline GridTune default_tune()
   t.update_pre = 0;
   t.comb_nv = 1;
   t.u2p_budget = 32;
+  t.u2p_group = 32;
   t.floor_nw = 4096;
   t.u2p_nw = 4096;
   return t;
@@ -1617,7 +1620,10 @@ void u2p(Ex &ex, const GridBufs &s, bool count_cleared = false)
     const OpU2P<T::MAG, T::RAD> op{s.P, s.G, s.u, s.p, s.fl, s.ulist, s.fcnt + 3, s.uwhy, s.tune.u2p_budget};
     ex.count(op, s.counters + 3);
     if (s.tune.floor_wave)
-      ex.count(OpU2PFloor<T::MAG, T::RAD, true>{op, s.tune.u2p_nw}, s.counters + 3);
+      if (s.tune.u2p_group == 32)
+        ex.count(OpU2PFloor<T::MAG, T::RAD, true, 32>{op, s.tune.u2p_nw}, s.counters + 3);
+      else
+        ex.count(OpU2PFloor<T::MAG, T::RAD, true, 64>{op, s.tune.u2p_nw}, s.counters + 3);
     else
       ex.count(OpU2PFloor<T::MAG, T::RAD, false>{op, s.tune.u2p_nw}, s.counters + 3);
   });

@@ -926,6 +926,7 @@ void tune_to_c(const GridTune &a, koral_grid_tune *t)
   t->update_pre = a.update_pre;
   t->comb_nv = a.comb_nv;
   t->u2p_budget = a.u2p_budget;
+  t->u2p_group = a.u2p_group;
   t->floor_nw = a.floor_nw;
   t->u2p_nw = a.u2p_nw;
 }
@@ -936,6 +937,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
     if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
       return set_error(KORAL_E_ARG, "set_tune: block sizes must be 64, 128 or 256");
   if (t->u2p_budget < 0) return set_error(KORAL_E_ARG, "set_tune: u2p_budget must be >= 0");
+  if (t->u2p_group != 32 && t->u2p_group != 64) return set_error(KORAL_E_ARG, "set_tune: u2p_group must be 32 or 64");
   if (t->floor_nw < 1 || t->floor_nw > (1L << 20) || t->u2p_nw < 1 || t->u2p_nw > (1L << 20))
     return set_error(KORAL_E_ARG, "set_tune: waves in flight must be in [1, 2^20]");
   for (int k = 0; k < TUNE_KINDS; k++) a.block[k] = t->block[k];
@@ -944,6 +946,7 @@ int tune_from_c(const koral_grid_tune *t, GridTune &a)
   a.update_pre = t->update_pre ? 1 : 0;
   a.comb_nv = t->comb_nv ? 1 : 0;
   a.u2p_budget = t->u2p_budget;
+  a.u2p_group = t->u2p_group;
   a.floor_nw = t->floor_nw;
   a.u2p_nw = t->u2p_nw;
   return 0;
@@ -1144,6 +1147,7 @@ int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, c
   coord("update_pre", [&] { return (long)T.update_pre; }, [&](long v) { T.update_pre = (int)v; }, {0, 1});
   coord("comb_nv", [&] { return (long)T.comb_nv; }, [&](long v) { T.comb_nv = (int)v; }, {1, 0});
   coord("u2p_budget", [&] { return (long)T.u2p_budget; }, [&](long v) { T.u2p_budget = (int)v; }, {32, 0, 16, 64});
+  coord("u2p_group", [&] { return (long)T.u2p_group; }, [&](long v) { T.u2p_group = (int)v; }, {32, 64});
   // block sizes of the operator kinds a step launches
   const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_BC_TOTAL, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
                        K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};

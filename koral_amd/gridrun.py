@@ -19,7 +19,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class GridTune(ctypes.Structure):
     """koral_grid_tune: the launch choices of a grid (include/koral_hip.h)."""
     _fields_ = [("block", ctypes.c_int * 32), ("fused", ctypes.c_int * 3), ("floor_wave", ctypes.c_int),
-                ("update_pre", ctypes.c_int), ("comb_nv", ctypes.c_int), ("u2p_budget", ctypes.c_int),
+                ("update_pre", ctypes.c_int), ("comb_nv", ctypes.c_int), ("u2p_budget", ctypes.c_int), ("u2p_group", ctypes.c_int),
                 ("floor_nw", ctypes.c_long),
                 ("u2p_nw", ctypes.c_long)]
 

@@ -353,6 +353,7 @@ def test_grid_launch_choices_bit_identical(kind):
         t.update_pre = fused
         t.comb_nv = wave
         t.u2p_budget = 3 if fused else 0
+        t.u2p_group = 32 if block == 128 else 64
         for k in range(32):
             t.block[k] = block
         t.floor_nw, t.u2p_nw = floor_nw, u2p_nw
