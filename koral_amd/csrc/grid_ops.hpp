@@ -185,22 +185,17 @@ struct EdgeGeo {
   KD_HD int c() const { return 3 - a - b; }
   KD_HD int ngc() const { return c() == 0 ? G.ngx : (c() == 1 ? G.ngy : G.ngz); }  // 0 if collapsed
   KD_HD int nk() const { return (c() == 0 ? G.nx : (c() == 1 ? G.ny : G.nz)) + 2 * ngc(); }
+  // index d of the cell at (va, vb) in the a-b plane and k along c, by
+  // selects (an array indexed by a and b would live in scratch memory)
+  KD_INL int coord(int d, int va, int vb, int k) const { return a == d ? va : (b == d ? vb : k); }
   KD_INL long cell(int va, int vb, int k) const
   {
-    int q[3];
-    q[a] = va;
-    q[b] = vb;
-    q[c()] = k;
-    return cidx(G, q[0], q[1], q[2]);
+    return cidx(G, coord(0, va, vb, k), coord(1, va, vb, k), coord(2, va, vb, k));
   }
   KD_INL void pos(int va, int vb, int k, int &x, int &y) const
   {
-    int q[3];
-    q[a] = va;
-    q[b] = vb;
-    q[c()] = k;
-    x = q[0];
-    y = q[1];
+    x = coord(0, va, vb, k);
+    y = coord(1, va, vb, k);
   }
 };
 
@@ -294,22 +289,47 @@ struct OpBcTotal {
   }
 };
 
-// the last edge pair (y-z, along x) and the corner blocks in one launch:
-// OpBcTotal reads interior cells only and overwrites every cell that the pair
-// writes at ghost positions along x, so those edge items are left out and the
-// launch has no internal dependence
+// the three edge pairs and the corner blocks of a 3-D grid in one launch.
+// At a position inside the domain along its third dimension an edge item
+// reads face ghosts only (written by OpBcFaces before this launch); at a
+// ghost position it would write a cell with all three indices outside the
+// domain, which OpBcTotal overwrites from an interior cell (p and u).  So
+// the edge items at ghost positions are left out, the final values are
+// those of the reference's order (all positions of the x-y, x-z and y-z
+// edges, then the corners), and the launch has no internal dependence.
 template <int MAG, int RAD>
-struct OpBcEdgeTotal {
+struct OpBcEdgesTotal {
   static constexpr int kind = K_BC_EDGE;
-  OpBcEdgeA<MAG, RAD> e;  // E = (1, 2), inner_k set
-  OpBcTotal<MAG, RAD> t;
-  KD_HD long size() const { return e.size() + t.size(); }
+  // one copy of the parameters and the grid: the sub-operators are built per
+  // item with constant edge axes (four copies as members made a 2 KiB
+  // argument block that the kernel copied into scratch memory)
+  koral_params P;
+  DGrid G;
+  double *p, *u;
+  KD_HD OpBcEdgeA<MAG, RAD> edge(int a, int b) const { return {P, EdgeGeo<MAG, RAD>{G, a, b}, p, u, true}; }
+  KD_HD long size() const
+  {
+    return edge(0, 1).size() + edge(0, 2).size() + edge(1, 2).size() + OpBcTotal<MAG, RAD>{P, G, p, u}.size();
+  }
   KD_INL void operator()(long i) const
   {
-    if (i < e.size())
-      e(i);
+    const long n01 = edge(0, 1).size();
+    if (i < n01) {
+      edge(0, 1)(i);
+      return;
+    }
+    i -= n01;
+    const long n02 = edge(0, 2).size();
+    if (i < n02) {
+      edge(0, 2)(i);
+      return;
+    }
+    i -= n02;
+    const long n12 = edge(1, 2).size();
+    if (i < n12)
+      edge(1, 2)(i);
     else
-      t(i - e.size());
+      OpBcTotal<MAG, RAD>{P, G, p, u}(i - n12);
   }
 };
 
@@ -1470,18 +1490,10 @@ void set_bc(Ex &ex, const GridBufs &s)
     using T = decltype(t);
     constexpr int M = T::MAG, R = T::RAD;
     ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u});
-    auto edges = [&](int a, int b) {
-      EdgeGeo<M, R> E{G, a, b};
-      ex.each(OpBcEdgeA<M, R>{s.P, E, s.p, s.u, false});
-    };
-    if (G.ny > 1) {
-      edges(0, 1);
-      if (G.nz > 1) {
-        edges(0, 2);
-        EdgeGeo<M, R> E{G, 1, 2};
-        ex.each(OpBcEdgeTotal<M, R>{OpBcEdgeA<M, R>{s.P, E, s.p, s.u, true}, OpBcTotal<M, R>{s.P, G, s.p, s.u}});
-      }
-    }
+    if (G.ny > 1 && G.nz > 1)
+      ex.each(OpBcEdgesTotal<M, R>{s.P, G, s.p, s.u});
+    else if (G.ny > 1)
+      ex.each(OpBcEdgeA<M, R>{s.P, EdgeGeo<M, R>{G, 0, 1}, s.p, s.u, false});
   });
 }
 
