@@ -94,6 +94,40 @@ struct OpP2U {
 // ---------------------------------------------------------------------------
 // ghost fill (set_bc finite.c:2752-3700)
 // ---------------------------------------------------------------------------
+// Primitives of the face-ghost cell (x, y, z), index d outside the domain and
+// the other two inside, as the face fill writes them: an exact copy of the
+// cell that the boundary type of that side maps it to, with the d components
+// of the vectors negated on a reflecting side.  Every coordinate and
+// component goes through a select over all three directions (a loop that
+// touches only index d is folded into a run-time index, and the array then
+// lives in scratch memory).
+template <int MAG, int RAD>
+KD_INL void face_ghost_p(const DGrid &G, const double *p, int d, int x, int y, int z, double *pv)
+{
+  const int gk = d == 0 ? x : (d == 1 ? y : z);
+  const int n = dimn(G, d), hi = gk >= n;
+  const int type = d == 0 ? (hi ? G.bc[1] : G.bc[0]) : (d == 1 ? (hi ? G.bc[3] : G.bc[2]) : (hi ? G.bc[5] : G.bc[4]));
+  int sk = gk;
+  bool flip = false;
+  if (type == BC_PERIODIC) {
+    if (sk < 0) sk += n;
+    if (sk > n - 1) sk -= n;
+  } else if (type == BC_REFLECT) {
+    sk = hi ? 2 * n - sk - 1 : -sk - 1;
+    flip = true;
+  } else {
+    sk = hi ? n - 1 : 0;
+  }
+  load_p<MAG, RAD>(p, G.ncell, cidx(G, d == 0 ? sk : x, d == 1 ? sk : y, d == 2 ? sk : z), pv);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const bool f = flip && k == d;
+    pv[2 + k] = f ? -pv[2 + k] : pv[2 + k];
+    if (MAG) pv[6 + k] = f ? -pv[6 + k] : pv[6 + k];
+    if (RAD) pv[Lay<MAG, RAD>::EE + 1 + k] = f ? -pv[Lay<MAG, RAD>::EE + 1 + k] : pv[Lay<MAG, RAD>::EE + 1 + k];
+  }
+}
+
 // ghost cells outside the domain in exactly one dimension
 template <int MAG, int RAD>
 struct OpBcFaces {
@@ -126,41 +160,11 @@ struct OpBcFaces {
     const Box b{0, 0, 0, d == 0 ? 2 * ng : G.nx, d == 1 ? 2 * ng : G.ny, d == 2 ? 2 * ng : G.nz};
     int ii[3];
     b.at(i, ii[0], ii[1], ii[2]);
-    // direction d through constant indices (k == d), so no array here is
-    // indexed at run time and none lives in scratch memory
-    int src[3];
-    bool flip = false;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (k == d) {
-        const int hi = ii[k] >= ng;
-        ii[k] = hi ? n + ii[k] - ng : ii[k] - ng;
-        int sk = ii[k];
-        const int type = G.bc[2 * k + hi];
-        if (type == BC_PERIODIC) {
-          if (sk < 0) sk += n;
-          if (sk > n - 1) sk -= n;
-        } else if (type == BC_REFLECT) {
-          sk = hi ? 2 * n - sk - 1 : -sk - 1;
-          flip = true;
-        } else {
-          sk = hi ? n - 1 : 0;
-        }
-        src[k] = sk;
-      } else {
-        src[k] = ii[k];
-      }
-    }
+    for (int k = 0; k < 3; k++) ii[k] = k != d ? ii[k] : (ii[k] >= ng ? n + ii[k] - ng : ii[k] - ng);
     double pv[NV];
-    load_p<MAG, RAD>(p, G.ncell, cidx(G, src[0], src[1], src[2]), pv);
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-      if (flip && k == d) {
-        pv[2 + k] = -pv[2 + k];
-        if (MAG) pv[6 + k] = -pv[6 + k];
-        if (RAD) pv[Lay<MAG, RAD>::EE + 1 + k] = -pv[Lay<MAG, RAD>::EE + 1 + k];
-      }
-    long c = cidx(G, ii[0], ii[1], ii[2]);
+    face_ghost_p<MAG, RAD>(G, p, d, ii[0], ii[1], ii[2], pv);
+    const long c = cidx(G, ii[0], ii[1], ii[2]);
     store_p<MAG, RAD>(p, G.ncell, c, pv);
     Geom g;
     geom_cell(G, ii[0], ii[1], g);
@@ -170,14 +174,14 @@ struct OpBcFaces {
   }
 };
 
-// Ghosts along the four a-b edges of the domain, at every position k along
-// the third dimension c: finite.c:3336-3600 recipe, in one launch.  The copies of the far face ghosts (NG-1 deep on each side of
-// the edge) and the average next to the corner, (ea, eb) = <(ea, ib),
-// (ia, eb)>, read face ghosts only.  The reference then averages two of those
-// copies on the diagonal, (ea2, eb2) = <(ea2, eb), (ea, eb2)>; the copies are
-// exact, so the same item reads their sources (ea2, ib) and (ia, eb2)
-// instead, with the same operand order, and the launch has no internal
-// dependence.  Each written cell gets u = p2u(p); NG >= 2.
+// Ghosts along the four a-b edges of the domain, at the positions k inside
+// the domain along the third dimension c: finite.c:3336-3600 recipe.  The
+// copies of the far face ghosts (NG-1 deep on each side of the edge) and the
+// average next to the corner, (ea, eb) = <(ea, ib), (ia, eb)>, read face
+// ghosts.  The reference then averages two of those copies on the diagonal,
+// (ea2, eb2) = <(ea2, eb), (ea, eb2)>; the copies are exact, so the same item
+// reads their sources (ea2, ib) and (ia, eb2) instead, with the same operand
+// order.  Each written cell gets u = p2u(p); NG >= 2.
 template <int MAG, int RAD>
 struct EdgeGeo {
   DGrid G;
@@ -199,22 +203,26 @@ struct EdgeGeo {
   }
 };
 
+// the edge cell (da, db) at k gets the face ghost (s1a, s1b) (index d1
+// outside the domain) or, averaging, <(s1a, s1b), (s2a, s2b)> with the
+// second an E.b face ghost; face ghosts are read through the face fill's map
+// from the cells inside, so this needs no earlier face fill
 template <int MAG, int RAD>
 KD_INL void edge_set(const koral_params &P, const EdgeGeo<MAG, RAD> &E, double *p, double *u, int k, int da, int db,
-                     int s1a, int s1b, int s2a, int s2b, bool average)
+                     int d1, int s1a, int s1b, int s2a, int s2b, bool average)
 {
   constexpr int NV = Lay<MAG, RAD>::NV;
   const long n = E.G.ncell;
-  const long d = E.cell(da, db, k), q1 = E.cell(s1a, s1b, k);
   double pv[NV];
+  face_ghost_p<MAG, RAD>(E.G, p, d1, E.coord(0, s1a, s1b, k), E.coord(1, s1a, s1b, k), E.coord(2, s1a, s1b, k), pv);
   if (average) {
-    const long q2 = E.cell(s2a, s2b, k);
+    double pw[NV];
+    face_ghost_p<MAG, RAD>(E.G, p, E.b, E.coord(0, s2a, s2b, k), E.coord(1, s2a, s2b, k), E.coord(2, s2a, s2b, k),
+                           pw);
 #pragma unroll
-    for (int iv = 0; iv < NV; iv++) pv[iv] = .5 * (p[iv * n + q1] + p[iv * n + q2]);
-  } else {
-#pragma unroll
-    for (int iv = 0; iv < NV; iv++) pv[iv] = p[iv * n + q1];
+    for (int iv = 0; iv < NV; iv++) pv[iv] = .5 * (pv[iv] + pw[iv]);
   }
+  const long d = E.cell(da, db, k);
   store_p<MAG, RAD>(p, n, d, pv);
   int x, y;
   E.pos(da, db, k, x, y);
@@ -231,16 +239,15 @@ struct OpBcEdgeA {
   koral_params P;
   EdgeGeo<MAG, RAD> E;
   double *p, *u;
-  bool inner_k;  // positions k along the edge: the interior ones only
   KD_HD int per() const { return 2 * (E.G.ng - 1) + 2; }  // items per (corner, k)
-  KD_HD int nkk() const { return inner_k ? E.nk() - 2 * E.ngc() : E.nk(); }
+  KD_HD int nkk() const { return E.nk() - 2 * E.ngc(); }  // positions inside along c
   KD_HD long size() const { return 4L * nkk() * per(); }
   KD_INL void operator()(long t) const
   {
     const int NG = E.G.ng;
     const int m = (int)(t % per());
     const long r = t / per();
-    const int k = (inner_k ? 0 : -E.ngc()) + (int)(r % nkk());
+    const int k = (int)(r % nkk());
     const int corner = (int)(r / nkk());
     const int ahi = corner >> 1, bhi = corner & 1;
     const int na = dimn(E.G, E.a), nb = dimn(E.G, E.b);
@@ -248,16 +255,16 @@ struct OpBcEdgeA {
     const int eb = bhi ? nb : -1, ib = bhi ? nb - 1 : 0;
     if (m < NG - 1) {
       const int ga = ahi ? na + m + 1 : -NG + m;
-      edge_set<MAG, RAD>(P, E, p, u, k, ga, eb, ga, ib, 0, 0, false);
+      edge_set<MAG, RAD>(P, E, p, u, k, ga, eb, E.a, ga, ib, 0, 0, false);
     } else if (m < 2 * (NG - 1)) {
       const int i = m - (NG - 1);
       const int gb = bhi ? nb + i + 1 : -NG + i;
-      edge_set<MAG, RAD>(P, E, p, u, k, ea, gb, ia, gb, 0, 0, false);
+      edge_set<MAG, RAD>(P, E, p, u, k, ea, gb, E.b, ia, gb, 0, 0, false);
     } else if (m == 2 * (NG - 1)) {
-      edge_set<MAG, RAD>(P, E, p, u, k, ea, eb, ea, ib, ia, eb, true);
+      edge_set<MAG, RAD>(P, E, p, u, k, ea, eb, E.a, ea, ib, ia, eb, true);
     } else {
       const int ea2 = ahi ? na + 1 : -2, eb2 = bhi ? nb + 1 : -2;
-      edge_set<MAG, RAD>(P, E, p, u, k, ea2, eb2, ea2, ib, ia, eb2, true);
+      edge_set<MAG, RAD>(P, E, p, u, k, ea2, eb2, E.a, ea2, ib, ia, eb2, true);
     }
   }
 };
@@ -289,30 +296,40 @@ struct OpBcTotal {
   }
 };
 
-// the three edge pairs and the corner blocks of a 3-D grid in one launch.
-// At a position inside the domain along its third dimension an edge item
-// reads face ghosts only (written by OpBcFaces before this launch); at a
-// ghost position it would write a cell with all three indices outside the
-// domain, which OpBcTotal overwrites from an interior cell (p and u).  So
-// the edge items at ghost positions are left out, the final values are
-// those of the reference's order (all positions of the x-y, x-z and y-z
-// edges, then the corners), and the launch has no internal dependence.
+// The whole ghost fill in one launch: the face ghosts, the edge ghosts at the
+// positions inside the domain along their third dimension, and on a 3-D grid
+// the corner blocks.  The edge items read face ghosts through the face map
+// (from cells inside the domain) and OpBcTotal reads cells inside only, so
+// the launch has no internal dependence.  The reference's order (faces, the
+// x-y, x-z and y-z edges at every position, then the corners) also writes the
+// edge items at ghost positions; each of those cells has all three indices
+// outside the domain and OpBcTotal overwrites it (p and u), so the final
+// values are the same.
 template <int MAG, int RAD>
-struct OpBcEdgesTotal {
-  static constexpr int kind = K_BC_EDGE;
-  // one copy of the parameters and the grid: the sub-operators are built per
-  // item with constant edge axes (four copies as members made a 2 KiB
-  // argument block that the kernel copied into scratch memory)
-  koral_params P;
-  DGrid G;
-  double *p, *u;
-  KD_HD OpBcEdgeA<MAG, RAD> edge(int a, int b) const { return {P, EdgeGeo<MAG, RAD>{G, a, b}, p, u, true}; }
+struct OpBcAll {
+  static constexpr int kind = K_BC_FACES;
+  // the face fill is the one member (the kernel argument itself): it selects
+  // grid fields by a run-time direction, which a local copy would keep in
+  // scratch memory; the edge and corner sub-operators index them by
+  // constants only and are built per item from its fields
+  OpBcFaces<MAG, RAD> f;
+  KD_HD OpBcEdgeA<MAG, RAD> edge(int a, int b) const { return {f.P, EdgeGeo<MAG, RAD>{f.G, a, b}, f.p, f.u}; }
+  KD_HD OpBcTotal<MAG, RAD> corners() const { return {f.P, f.G, f.p, f.u}; }
+  KD_HD bool has2() const { return f.G.ny > 1; }
+  KD_HD bool has3() const { return f.G.ny > 1 && f.G.nz > 1; }
   KD_HD long size() const
   {
-    return edge(0, 1).size() + edge(0, 2).size() + edge(1, 2).size() + OpBcTotal<MAG, RAD>{P, G, p, u}.size();
+    return f.size() + (has2() ? edge(0, 1).size() : 0) +
+           (has3() ? edge(0, 2).size() + edge(1, 2).size() + corners().size() : 0);
   }
   KD_INL void operator()(long i) const
   {
+    const long nf = f.size();
+    if (i < nf) {
+      f(i);
+      return;
+    }
+    i -= nf;
     const long n01 = edge(0, 1).size();
     if (i < n01) {
       edge(0, 1)(i);
@@ -329,7 +346,7 @@ struct OpBcEdgesTotal {
     if (i < n12)
       edge(1, 2)(i);
     else
-      OpBcTotal<MAG, RAD>{P, G, p, u}(i - n12);
+      corners()(i - n12);
   }
 };
 
@@ -1485,15 +1502,9 @@ void grid_p2u(Ex &ex, const GridBufs &s, bool interior_only = false)
 template <class Ex>
 void set_bc(Ex &ex, const GridBufs &s)
 {
-  const DGrid &G = s.G;
   dispatch(s.P, [&](auto t) {
     using T = decltype(t);
-    constexpr int M = T::MAG, R = T::RAD;
-    ex.each(OpBcFaces<M, R>{s.P, G, s.p, s.u});
-    if (G.ny > 1 && G.nz > 1)
-      ex.each(OpBcEdgesTotal<M, R>{s.P, G, s.p, s.u});
-    else if (G.ny > 1)
-      ex.each(OpBcEdgeA<M, R>{s.P, EdgeGeo<M, R>{G, 0, 1}, s.p, s.u, false});
+    ex.each(OpBcAll<T::MAG, T::RAD>{OpBcFaces<T::MAG, T::RAD>{s.P, s.G, s.p, s.u}});
   });
 }
 

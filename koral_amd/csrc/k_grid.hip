@@ -1149,7 +1149,7 @@ int koral_hip_grid_autotune(koral_hip_grid *g, double t, double dt, int steps, c
   coord("u2p_budget", [&] { return (long)T.u2p_budget; }, [&](long v) { T.u2p_budget = (int)v; }, {32, 0, 16, 64});
   coord("u2p_group", [&] { return (long)T.u2p_group; }, [&](long v) { T.u2p_group = (int)v; }, {32, 64});
   // block sizes of the operator kinds a step launches
-  const int kinds[] = {K_BC_FACES, K_BC_EDGE, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
+  const int kinds[] = {K_BC_FACES, K_WAVESPEEDS, K_RECON, K_FLUX, K_FLUX_FLOOR, K_EMF, K_CT,
                        K_UPDATE, K_U2P, K_U2P_FLOOR, K_FIXUP, K_FIXUP_APPLY, K_COMB, K_POLAR};
   for (int k : kinds) {
     snprintf(line, sizeof line, "block_%s", koral_hip_grid_op_name(k));
