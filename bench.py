@@ -397,7 +397,7 @@ def grid_alg_bytes(kind, nv, fused=False):
         "wavespeeds": (nv + 18) * 8,               # per cell: prims, 18 speeds
         "emf": 9 * 8, "ct_faces": 9 * 8,           # per edge
         "comb": 3 * 8,                             # per (cell, variable)
-        "bc_faces": (3 * nv) * 8, "bc_edge": 4 * (3 * nv) * 8, "bc_total": (3 * nv) * 8,
+        "bc_faces": (3 * nv) * 8,                  # per ghost cell (the whole fill): source prims, p and u
         "p2u": (2 * nv) * 8,
         "fixup": 8, "fixup_apply": 8,
         "implicit": (4 * nv + 40 + 1) * 8 + 3,
