@@ -48,6 +48,10 @@ def _setup(kind):
     if kind == "tube":
         P = KP.params_tube()
         spec, p = PR.tube_setup(P, nx=64)
+    elif kind == "torus_mhd_2d":
+        # axisymmetric grid (one x3 cell): faces and the x1-x2 edges only
+        P = KP.params_torus_mhd()
+        spec, p = PR.torus_setup(P, n=(16, 12, 1))
     elif kind in ("torus_mhd", "torus_mhd_polar", "torus_mhd_bfloor"):
         P = KP.params_torus_mhd()
         if kind == "torus_mhd_polar":
@@ -133,7 +137,7 @@ def test_grid_explicit_parity(oracle, kind):
 
 
 @pytest.mark.parametrize("kind,nsteps", [("tube", 5), ("torus_mhd", 2), ("torus_mhd_polar", 2), ("torus_mhd_bfloor", 2),
-                                         pytest.param("torus_rad", 2, marks=RAD_TORUS_OFF)])
+                                         ("torus_mhd_2d", 2), pytest.param("torus_rad", 2, marks=RAD_TORUS_OFF)])
 def test_grid_step_parity(oracle, kind, nsteps):
     from koral_amd.gridrun import DeviceGrid
     P, spec, p = _setup(kind)

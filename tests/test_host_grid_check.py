@@ -29,6 +29,9 @@ def _setup(kind):
     elif kind == "torus_mhd":
         P = KP.params_torus_mhd()
         spec, p = PR.torus_setup(P, n=(16, 16, 8))
+    elif kind == "torus_mhd_2d":
+        P = KP.params_torus_mhd()
+        spec, p = PR.torus_setup(P, n=(16, 12, 1))
     elif kind == "torus_mhd_odd":
         P = KP.params_torus_mhd()
         spec, p = PR.torus_setup(P, n=(13, 10, 6))
@@ -48,7 +51,7 @@ def _same(a, b):
 KINDS = ["tube", "torus_mhd", "torus_mhd_odd", "torus_rad", "torus_rad_nopolar", "torus_rad_odd"]
 
 
-@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kind", KINDS + ["torus_mhd_2d"])
 def test_host_step_matches_oracle(oracle, exe, tmp_path, kind):
     P, spec, p = _setup(kind)
     nsteps = 4 if kind == "tube" else 2
